@@ -1,0 +1,168 @@
+/*
+ * ochre.h -- C ABI of the MI355X (gfx950) implementation of ochre's
+ * rasterisation path: Rasterizer::fill(&[PathCmd], Transform) ->
+ * Rasterizer::finish(&mut impl TileBuilder) emitting 8x8 u8 alpha tiles and
+ * solid n x 8 spans.
+ *
+ * Reference interface each entry point replaces (glowcoil/ochre v0.2.0):
+ *   ochre_rasterizer_new        Rasterizer::new        src/rasterizer.rs:50-58
+ *   ochre_rasterizer_move_to    Rasterizer::move_to    src/rasterizer.rs:61-69
+ *   ochre_rasterizer_line_to    Rasterizer::line_to    src/rasterizer.rs:72-140
+ *   ochre_rasterizer_command    Rasterizer::command    src/rasterizer.rs:145-157
+ *   ochre_rasterizer_fill       Rasterizer::fill       src/rasterizer.rs:161-165
+ *   ochre_rasterizer_finish     Rasterizer::finish     src/rasterizer.rs:180-268
+ *   ochre_tile_builder          trait TileBuilder      src/rasterizer.rs:12-22
+ *   ochre_transform             Transform / Mat2x2     src/geom.rs:131-133,203-207
+ *   OCHRE_VERB_*                enum PathCmd           src/path.rs:5-12
+ * The batch entry points (ochre_rasterize_plain / ochre_rasterize_batched)
+ * run many independent Rasterizers (one per path, as examples/svg.rs:143-145
+ * does) on device-resident inputs; they have no single reference
+ * counterpart and are the throughput interface.
+ *
+ * Paths are passed structure-of-arrays: one verb byte per PathCmd, the
+ * command points as consecutive f32 (x, y) pairs (Move/Line 1, Quadratic 2,
+ * Cubic 3, Conic 2, Close 0), and one weight per Conic.
+ *
+ * Errors: every entry point returns an int status (0 = OCHRE_OK, negative
+ * on failure); nothing unwinds across the ABI.  The reference API is
+ * infallible; failures here are argument, memory or device errors.
+ */
+#ifndef OCHRE_H
+#define OCHRE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OCHRE_TILE_SIZE 8
+
+enum ochre_verb {
+    OCHRE_VERB_MOVE = 0,
+    OCHRE_VERB_LINE = 1,
+    OCHRE_VERB_QUADRATIC = 2,
+    OCHRE_VERB_CUBIC = 3,
+    OCHRE_VERB_CONIC = 4,
+    OCHRE_VERB_CLOSE = 5
+};
+
+enum ochre_status {
+    OCHRE_OK = 0,
+    OCHRE_ERR_INVALID_ARG = -1,
+    OCHRE_ERR_OUT_OF_MEMORY = -2,
+    OCHRE_ERR_DEVICE = -3,
+    OCHRE_ERR_CAPACITY = -4,      /* output arrays too small; required sizes reported */
+    OCHRE_ERR_FLATTEN_LIMIT = -5, /* a curve needed > 65536 lines or conic depth > 32
+                                     (inputs on which the reference does not terminate) */
+    OCHRE_ERR_NO_DEVICE = -6,
+    OCHRE_ERR_TOO_LARGE = -7      /* a batch exceeds 32-bit internal indexing */
+};
+
+/* Transform (src/geom.rs:203-207): m is the row-major Mat2x2
+ * (m[0] m[1]; m[2] m[3]), applied as (m0*x + m1*y + ox, m2*x + m3*y + oy). */
+typedef struct ochre_transform {
+    float m[4];
+    float offset[2];
+} ochre_transform;
+
+/* TileBuilder (src/rasterizer.rs:12-22).  `alpha` is 64 bytes, row-major
+ * (alpha[row * 8 + col]), valid only during the call.  Calls arrive in the
+ * reference's order: row-major by (tile y, tile x), each span right after the
+ * tile to its left. */
+typedef struct ochre_tile_builder {
+    void* user;
+    void (*tile)(void* user, int16_t x, int16_t y, const uint8_t* alpha);
+    void (*span)(void* user, int16_t x, int16_t y, uint16_t width);
+} ochre_tile_builder;
+
+/* ------------------------------------------------------------ Rasterizer */
+
+typedef struct ochre_rasterizer ochre_rasterizer;
+
+/* Rasterizer::new.  Returns NULL on allocation failure or when no GPU is
+ * usable (the library has no CPU fallback). */
+ochre_rasterizer* ochre_rasterizer_new(void);
+int ochre_rasterizer_move_to(ochre_rasterizer* r, float x, float y);
+int ochre_rasterizer_line_to(ochre_rasterizer* r, float x, float y);
+/* `points` holds the command's points (verb-dependent count); `weight` is
+ * read for conics only.  No transform is applied (as Rasterizer::command). */
+int ochre_rasterizer_command(ochre_rasterizer* r, uint8_t verb, const float* points, float weight);
+int ochre_rasterizer_fill(ochre_rasterizer* r, const uint8_t* verbs, size_t n_verbs, const float* points,
+                          size_t n_points, const float* weights, size_t n_weights,
+                          const ochre_transform* transform);
+/* Rasterizer::finish: rasterises on the GPU, then calls the builder.
+ * Consumes the rasterizer (the handle is freed, whatever the status). */
+int ochre_rasterizer_finish(ochre_rasterizer* r, const ochre_tile_builder* builder);
+/* Frees a rasterizer that will not be finished. */
+void ochre_rasterizer_free(ochre_rasterizer* r);
+
+/* -------------------------------------------------------- batch interface */
+
+/* Device-resident input: n_paths paths, each filled with its own transform.
+ * All pointers are device pointers; *_off arrays have n_paths + 1 entries
+ * (offsets in elements: verbs, points (pairs), weights). */
+typedef struct ochre_path_batch {
+    int64_t n_paths;
+    const uint8_t* verbs;
+    const int64_t* verb_off;
+    const float* points;
+    const int64_t* point_off;
+    const float* weights;
+    const int64_t* weight_off;
+    const ochre_transform* transforms;
+    int64_t n_verbs;   /* = verb_off[n_paths] (host copy) */
+    int64_t n_points;  /* = point_off[n_paths] */
+    int64_t n_weights; /* = weight_off[n_paths] */
+} ochre_path_batch;
+
+/* Device-resident output, caller-owned.  Tiles of path p occupy
+ * [path_tile_start[p], path_tile_start[p] + path_tile_count[p]) in row-major
+ * (y, x) order (likewise spans); tile_alpha is 64 bytes per tile.  On
+ * OCHRE_ERR_CAPACITY nothing is written except n_tiles / n_spans, which
+ * hold the required sizes. */
+typedef struct ochre_batch_output {
+    int16_t* tile_xy;        /* [tile_capacity][2] */
+    uint8_t* tile_alpha;     /* [tile_capacity][64] */
+    int64_t tile_capacity;
+    int16_t* span_xy;        /* [span_capacity][2] */
+    uint16_t* span_width;    /* [span_capacity] */
+    int64_t span_capacity;
+    int64_t* path_tile_start; /* [n_paths] */
+    int32_t* path_tile_count; /* [n_paths] */
+    int64_t* path_span_start; /* [n_paths] */
+    int32_t* path_span_count; /* [n_paths] */
+    int64_t n_tiles;         /* out */
+    int64_t n_spans;         /* out */
+} ochre_batch_output;
+
+typedef struct ochre_context ochre_context;
+
+/* One context per device and thread; it owns scratch memory that grows on
+ * demand and is reused across calls. */
+int ochre_context_create(int device, ochre_context** out);
+void ochre_context_destroy(ochre_context* ctx);
+
+/* The five-kernel pipeline: (a) flatten, (b) DDA binning + device radix
+ * sort, (c)-(e) per-tile accumulation, row scan and compaction.  Paths are
+ * output contiguously in path order.  `stream` is a hipStream_t (NULL =
+ * default stream).  Synchronises the stream to size its scratch; outputs are
+ * complete when the stream is idle. */
+int ochre_rasterize_plain(ochre_context* ctx, const ochre_path_batch* in, ochre_batch_output* out,
+                          void* stream);
+
+/* Kernel-time accounting for measurement: when enabled, the pipeline
+ * records HIP events around its dominant kernel; ochre_context_timing
+ * returns the summed milliseconds and launch count since the last reset. */
+int ochre_context_set_timing(ochre_context* ctx, int enabled);
+int ochre_context_timing(ochre_context* ctx, const char* kernel, double* total_ms, int64_t* launches);
+
+/* Version / build information (NUL-terminated, static storage). */
+const char* ochre_build_info(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

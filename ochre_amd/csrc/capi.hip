@@ -1,0 +1,381 @@
+// C ABI (include/ochre.h) and the host-side Rasterizer.
+//
+// The host Rasterizer records commands exactly as the reference's
+// Rasterizer receives them (fill: one segment per call with its transform;
+// move_to / line_to / command: untransformed segments) and defers all
+// arithmetic to the device pipeline at finish().  Nothing is rasterised on
+// the CPU: without a usable GPU, ochre_rasterizer_new returns NULL and the
+// batch entry points return OCHRE_ERR_NO_DEVICE.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <vector>
+
+#include "../../include/ochre.h"
+#include "pipeline.hpp"
+
+using namespace ochre;
+
+struct ochre_context {
+    Context c;
+    std::mutex mu;
+};
+
+static_assert(sizeof(ochre_transform) == sizeof(Xform), "transform layout");
+
+namespace {
+
+bool device_usable(int device) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= device || device < 0) return false;
+    return true;
+}
+
+std::mutex g_default_mu;
+ochre_context* g_default = nullptr;
+
+ochre_context* default_context() {
+    std::lock_guard<std::mutex> lk(g_default_mu);
+    if (!g_default) {
+        if (!device_usable(0)) return nullptr;
+        ochre_context* c = new (std::nothrow) ochre_context();
+        if (!c) return nullptr;
+        c->c.device = 0;
+        g_default = c;
+    }
+    return g_default;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* ochre_build_info(void) {
+    return "ochre-mi355x: gfx950 HIP kernels (flatten, DDA binning, radix sort, tile accumulation, row scan, "
+           "compaction)";
+}
+
+int ochre_context_create(int device, ochre_context** out) {
+    if (!out) return OCHRE_ERR_INVALID_ARG;
+    *out = nullptr;
+    if (!device_usable(device)) return OCHRE_ERR_NO_DEVICE;
+    ochre_context* c = new (std::nothrow) ochre_context();
+    if (!c) return OCHRE_ERR_OUT_OF_MEMORY;
+    c->c.device = device;
+    *out = c;
+    return OCHRE_OK;
+}
+
+void ochre_context_destroy(ochre_context* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->c.device);
+    delete ctx;
+}
+
+int ochre_context_set_timing(ochre_context* ctx, int enabled) {
+    if (!ctx) return OCHRE_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->c.timing = enabled != 0;
+    ctx->c.timings.clear();
+    return OCHRE_OK;
+}
+
+int ochre_context_timing(ochre_context* ctx, const char* kernel, double* total_ms, int64_t* launches) {
+    if (!ctx || !kernel) return OCHRE_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    auto it = ctx->c.timings.find(kernel);
+    if (total_ms) *total_ms = it == ctx->c.timings.end() ? 0.0 : it->second.first;
+    if (launches) *launches = it == ctx->c.timings.end() ? 0 : it->second.second;
+    return OCHRE_OK;
+}
+
+int ochre_rasterize_plain(ochre_context* ctx, const ochre_path_batch* in, ochre_batch_output* out, void* stream) {
+    if (!ctx || !in || !out) return OCHRE_ERR_INVALID_ARG;
+    if (in->n_paths < 0 || in->n_verbs < 0 || in->n_points < 0 || in->n_weights < 0) return OCHRE_ERR_INVALID_ARG;
+    if (in->n_paths > 0 && (!in->verb_off || !in->point_off || !in->weight_off || !in->transforms))
+        return OCHRE_ERR_INVALID_ARG;
+    if (in->n_verbs > 0 && (!in->verbs)) return OCHRE_ERR_INVALID_ARG;
+    if (in->n_points > 0 && !in->points) return OCHRE_ERR_INVALID_ARG;
+    if (in->n_weights > 0 && !in->weights) return OCHRE_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (hipSetDevice(ctx->c.device) != hipSuccess) return OCHRE_ERR_DEVICE;
+    SegInput si;
+    si.n_segs = in->n_paths;
+    si.n_paths = in->n_paths;
+    si.verbs = in->verbs;
+    si.points = in->points;
+    si.weights = in->weights;
+    si.seg_verb_off = in->verb_off;
+    si.seg_point_off = in->point_off;
+    si.seg_weight_off = in->weight_off;
+    si.seg_xf = reinterpret_cast<const Xform*>(in->transforms);
+    si.n_verbs = in->n_verbs;
+    si.n_points = in->n_points;
+    si.n_weights = in->n_weights;
+    OutputDesc od;
+    od.tile_xy = out->tile_xy;
+    od.tile_alpha = out->tile_alpha;
+    od.tile_capacity = out->tile_capacity;
+    od.span_xy = out->span_xy;
+    od.span_w = out->span_width;
+    od.span_capacity = out->span_capacity;
+    od.path_tile_start = out->path_tile_start;
+    od.path_tile_count = out->path_tile_count;
+    od.path_span_start = out->path_span_start;
+    od.path_span_count = out->path_span_count;
+    int st = run_plain(&ctx->c, si, od, reinterpret_cast<hipStream_t>(stream));
+    out->n_tiles = od.n_tiles;
+    out->n_spans = od.n_spans;
+    return st;
+}
+
+}  // extern "C"
+
+// --------------------------------------------------------------- Rasterizer
+
+struct ochre_rasterizer {
+    std::vector<uint8_t> verbs;
+    std::vector<float> points;
+    std::vector<float> weights;
+    std::vector<int64_t> seg_verb_off{0}, seg_point_off{0}, seg_weight_off{0};
+    std::vector<Xform> seg_xf;
+    std::vector<uint8_t> seg_raw;
+    ochre_context* ctx = nullptr;
+
+    void open_segment(bool raw, const Xform& xf) {
+        seg_verb_off.push_back((int64_t)verbs.size());
+        seg_point_off.push_back((int64_t)points.size() / 2);
+        seg_weight_off.push_back((int64_t)weights.size());
+        seg_xf.push_back(xf);
+        seg_raw.push_back(raw ? 1 : 0);
+    }
+    // Segment ends are the next segment's starts; the trailing entries are
+    // kept equal to the current sizes.
+    void close_segment() {
+        seg_verb_off.back() = (int64_t)verbs.size();
+        seg_point_off.back() = (int64_t)points.size() / 2;
+        seg_weight_off.back() = (int64_t)weights.size();
+    }
+    void raw_command(uint8_t verb, const float* p, int np, float w) {
+        if (seg_raw.empty() || !seg_raw.back()) {
+            Xform id{1, 0, 0, 1, 0, 0};
+            // seg_*_off holds n_segs + 1 entries: the last one is the end.
+            seg_xf.push_back(id);
+            seg_raw.push_back(1);
+            seg_verb_off.push_back((int64_t)verbs.size());
+            seg_point_off.push_back((int64_t)points.size() / 2);
+            seg_weight_off.push_back((int64_t)weights.size());
+        }
+        verbs.push_back(verb);
+        for (int k = 0; k < 2 * np; k++) points.push_back(p[k]);
+        if (verb == OCHRE_VERB_CONIC) weights.push_back(w);
+        close_segment();
+    }
+};
+
+namespace {
+
+int verb_npts(uint8_t v) { return ochre::verb_points(v); }
+
+struct DevPath {
+    DBuf verbs, points, weights, seg_voff, seg_poff, seg_woff, seg_xf, seg_raw, seg_path, path_seg;
+    DBuf tile_xy, tile_alpha, span_xy, span_w, pts, ptc, pss, psc;
+};
+
+template <typename T>
+int upload(DBuf& b, const std::vector<T>& v, hipStream_t s) {
+    hipError_t e = b.ensure(std::max<size_t>(v.size() * sizeof(T), 16));
+    if (e != hipSuccess) return map_hip_error(e);
+    if (!v.empty()) {
+        e = hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s);
+        if (e != hipSuccess) return map_hip_error(e);
+    }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+ochre_rasterizer* ochre_rasterizer_new(void) {
+    ochre_context* ctx = default_context();
+    if (!ctx) return nullptr;
+    ochre_rasterizer* r = new (std::nothrow) ochre_rasterizer();
+    if (!r) return nullptr;
+    r->ctx = ctx;
+    return r;
+}
+
+void ochre_rasterizer_free(ochre_rasterizer* r) { delete r; }
+
+int ochre_rasterizer_move_to(ochre_rasterizer* r, float x, float y) {
+    if (!r) return OCHRE_ERR_INVALID_ARG;
+    float p[2] = {x, y};
+    r->raw_command(OCHRE_VERB_MOVE, p, 1, 0.0f);
+    return OCHRE_OK;
+}
+
+int ochre_rasterizer_line_to(ochre_rasterizer* r, float x, float y) {
+    if (!r) return OCHRE_ERR_INVALID_ARG;
+    float p[2] = {x, y};
+    r->raw_command(OCHRE_VERB_LINE, p, 1, 0.0f);
+    return OCHRE_OK;
+}
+
+int ochre_rasterizer_command(ochre_rasterizer* r, uint8_t verb, const float* points, float weight) {
+    if (!r) return OCHRE_ERR_INVALID_ARG;
+    int np = verb_npts(verb);
+    if (np < 0 || (np > 0 && !points)) return OCHRE_ERR_INVALID_ARG;
+    r->raw_command(verb, points, np, weight);
+    return OCHRE_OK;
+}
+
+int ochre_rasterizer_fill(ochre_rasterizer* r, const uint8_t* verbs, size_t n_verbs, const float* points,
+                          size_t n_points, const float* weights, size_t n_weights,
+                          const ochre_transform* transform) {
+    if (!r || !transform) return OCHRE_ERR_INVALID_ARG;
+    if ((n_verbs && !verbs) || (n_points && !points) || (n_weights && !weights)) return OCHRE_ERR_INVALID_ARG;
+    size_t np = 0, nw = 0;
+    for (size_t i = 0; i < n_verbs; i++) {
+        int k = verb_npts(verbs[i]);
+        if (k < 0) return OCHRE_ERR_INVALID_ARG;
+        np += (size_t)k;
+        nw += verbs[i] == OCHRE_VERB_CONIC;
+    }
+    if (np != n_points || nw != n_weights) return OCHRE_ERR_INVALID_ARG;
+    Xform xf;
+    memcpy(&xf, transform, sizeof(xf));
+    // open a transformed segment
+    r->seg_xf.push_back(xf);
+    r->seg_raw.push_back(0);
+    r->seg_verb_off.push_back((int64_t)r->verbs.size());
+    r->seg_point_off.push_back((int64_t)r->points.size() / 2);
+    r->seg_weight_off.push_back((int64_t)r->weights.size());
+    r->verbs.insert(r->verbs.end(), verbs, verbs + n_verbs);
+    r->points.insert(r->points.end(), points, points + 2 * n_points);
+    r->weights.insert(r->weights.end(), weights, weights + n_weights);
+    r->close_segment();
+    return OCHRE_OK;
+}
+
+int ochre_rasterizer_finish(ochre_rasterizer* r, const ochre_tile_builder* builder) {
+    if (!r) return OCHRE_ERR_INVALID_ARG;
+    ochre_context* ctx = r->ctx;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    static DevPath* dp = nullptr;  // guarded by the default context's mutex
+    if (!dp) dp = new (std::nothrow) DevPath();
+    int st = 0;
+    do {
+        if (!dp) {
+            st = OCHRE_ERR_OUT_OF_MEMORY;
+            break;
+        }
+        if (hipSetDevice(ctx->c.device) != hipSuccess) {
+            st = OCHRE_ERR_DEVICE;
+            break;
+        }
+        hipStream_t s = nullptr;
+        // a path with no segments is one empty segment
+        if (r->seg_xf.empty()) {
+            Xform id{1, 0, 0, 1, 0, 0};
+            r->seg_xf.push_back(id);
+            r->seg_raw.push_back(1);
+            r->seg_verb_off.push_back(0);
+            r->seg_point_off.push_back(0);
+            r->seg_weight_off.push_back(0);
+        }
+        const int64_t nseg = (int64_t)r->seg_xf.size();
+        std::vector<int32_t> seg_path((size_t)nseg, 0);
+        std::vector<int64_t> path_seg{0, nseg};
+        if ((st = upload(dp->verbs, r->verbs, s)) || (st = upload(dp->points, r->points, s)) ||
+            (st = upload(dp->weights, r->weights, s)) || (st = upload(dp->seg_voff, r->seg_verb_off, s)) ||
+            (st = upload(dp->seg_poff, r->seg_point_off, s)) || (st = upload(dp->seg_woff, r->seg_weight_off, s)) ||
+            (st = upload(dp->seg_xf, r->seg_xf, s)) || (st = upload(dp->seg_raw, r->seg_raw, s)) ||
+            (st = upload(dp->seg_path, seg_path, s)) || (st = upload(dp->path_seg, path_seg, s)))
+            break;
+        SegInput si;
+        si.n_segs = nseg;
+        si.n_paths = 1;
+        si.verbs = dp->verbs.as<uint8_t>();
+        si.points = dp->points.as<float>();
+        si.weights = dp->weights.as<float>();
+        si.seg_verb_off = dp->seg_voff.as<int64_t>();
+        si.seg_point_off = dp->seg_poff.as<int64_t>();
+        si.seg_weight_off = dp->seg_woff.as<int64_t>();
+        si.seg_xf = dp->seg_xf.as<Xform>();
+        si.seg_raw = dp->seg_raw.as<uint8_t>();
+        si.seg_path = dp->seg_path.as<int32_t>();
+        si.path_seg_off = dp->path_seg.as<int64_t>();
+        si.n_verbs = (int64_t)r->verbs.size();
+        si.n_points = (int64_t)r->points.size() / 2;
+        si.n_weights = (int64_t)r->weights.size();
+        OutputDesc od;
+        int64_t tcap = (int64_t)(dp->tile_alpha.bytes / 64), scap = (int64_t)(dp->span_w.bytes / 2);
+        for (int attempt = 0; attempt < 2; attempt++) {
+            hipError_t e;
+            if ((e = dp->tile_xy.ensure(std::max<int64_t>(tcap, 64) * 4)) != hipSuccess ||
+                (e = dp->tile_alpha.ensure(std::max<int64_t>(tcap, 64) * 64)) != hipSuccess ||
+                (e = dp->span_xy.ensure(std::max<int64_t>(scap, 64) * 4)) != hipSuccess ||
+                (e = dp->span_w.ensure(std::max<int64_t>(scap, 64) * 2)) != hipSuccess ||
+                (e = dp->pts.ensure(8)) != hipSuccess || (e = dp->ptc.ensure(8)) != hipSuccess ||
+                (e = dp->pss.ensure(8)) != hipSuccess || (e = dp->psc.ensure(8)) != hipSuccess) {
+                st = map_hip_error(e);
+                break;
+            }
+            od.tile_xy = dp->tile_xy.as<int16_t>();
+            od.tile_alpha = dp->tile_alpha.as<uint8_t>();
+            od.tile_capacity = (int64_t)(dp->tile_alpha.bytes / 64);
+            od.span_xy = dp->span_xy.as<int16_t>();
+            od.span_w = dp->span_w.as<uint16_t>();
+            od.span_capacity = (int64_t)(dp->span_w.bytes / 2);
+            od.path_tile_start = dp->pts.as<int64_t>();
+            od.path_tile_count = dp->ptc.as<int32_t>();
+            od.path_span_start = dp->pss.as<int64_t>();
+            od.path_span_count = dp->psc.as<int32_t>();
+            st = run_plain(&ctx->c, si, od, s);
+            if (st != OCHRE_ERR_CAPACITY) break;
+            tcap = od.n_tiles;
+            scap = od.n_spans;
+        }
+        if (st) break;
+        std::vector<int16_t> txy((size_t)od.n_tiles * 2), sxy((size_t)od.n_spans * 2);
+        std::vector<uint8_t> ta((size_t)od.n_tiles * 64);
+        std::vector<uint16_t> sw((size_t)od.n_spans);
+        hipError_t e = hipSuccess;
+        if (od.n_tiles) {
+            if ((e = hipMemcpyAsync(txy.data(), od.tile_xy, txy.size() * 2, hipMemcpyDeviceToHost, s)) ||
+                (e = hipMemcpyAsync(ta.data(), od.tile_alpha, ta.size(), hipMemcpyDeviceToHost, s))) {
+            }
+        }
+        if (e == hipSuccess && od.n_spans) {
+            if ((e = hipMemcpyAsync(sxy.data(), od.span_xy, sxy.size() * 2, hipMemcpyDeviceToHost, s)) ||
+                (e = hipMemcpyAsync(sw.data(), od.span_w, sw.size() * 2, hipMemcpyDeviceToHost, s))) {
+            }
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) {
+            st = map_hip_error(e);
+            break;
+        }
+        // TileBuilder replay in the reference's call order: a span follows
+        // the tile to its left (rasterizer.rs:241,263).
+        if (builder) {
+            size_t si_ = 0;
+            for (int64_t t = 0; t < od.n_tiles; t++) {
+                int16_t x = txy[2 * t], y = txy[2 * t + 1];
+                if (builder->tile) builder->tile(builder->user, x, y, &ta[(size_t)t * 64]);
+                while (si_ < (size_t)od.n_spans && sxy[2 * si_ + 1] == y && sxy[2 * si_] == (int16_t)(x + 8)) {
+                    if (builder->span) builder->span(builder->user, sxy[2 * si_], sxy[2 * si_ + 1], sw[si_]);
+                    si_++;
+                }
+            }
+        }
+    } while (0);
+    delete r;
+    return st;
+}
+
+}  // extern "C"

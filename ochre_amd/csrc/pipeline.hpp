@@ -1,0 +1,130 @@
+// Internal interfaces of the MI355X rasterisation pipeline.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "device_common.hpp"
+#include "scan.hpp"
+
+namespace ochre {
+
+// Device input.  A path is one Rasterizer; it consists of one or more
+// segments, each a run of commands filled with one transform (one segment
+// per Rasterizer::fill call; `raw` segments come from move_to / line_to /
+// command and are not transformed).  The batch API maps path i to segment i.
+struct SegInput {
+    int64_t n_segs = 0;
+    int64_t n_paths = 0;
+    const uint8_t* verbs = nullptr;
+    const float* points = nullptr;    // (x, y) pairs
+    const float* weights = nullptr;
+    const int64_t* seg_verb_off = nullptr;   // [n_segs + 1]
+    const int64_t* seg_point_off = nullptr;  // [n_segs + 1]
+    const int64_t* seg_weight_off = nullptr; // [n_segs + 1]
+    const Xform* seg_xf = nullptr;           // [n_segs]
+    const uint8_t* seg_raw = nullptr;        // [n_segs] or null (none raw)
+    const int32_t* seg_path = nullptr;       // [n_segs] or null (segment i -> path i)
+    const int64_t* path_seg_off = nullptr;   // [n_paths + 1] or null (identity)
+    int64_t n_verbs = 0, n_points = 0, n_weights = 0;
+};
+
+struct OutputDesc {
+    int16_t* tile_xy = nullptr;
+    uint8_t* tile_alpha = nullptr;
+    int64_t tile_capacity = 0;
+    int16_t* span_xy = nullptr;
+    uint16_t* span_w = nullptr;
+    int64_t span_capacity = 0;
+    int64_t* path_tile_start = nullptr;
+    int32_t* path_tile_count = nullptr;
+    int64_t* path_span_start = nullptr;
+    int32_t* path_span_count = nullptr;
+    int64_t n_tiles = 0, n_spans = 0;
+};
+
+// Per-path tile grid: keys are ((row_base + ty - ty0) << xbits) | (tx - tx0).
+struct PathGrid {
+    int32_t tx0, ty0, w, h;
+    uint32_t row_base;
+    int32_t pad[3];
+};
+
+// Per-path bounding box of everything the DDA can visit (pixel coords).
+struct PathBox {
+    int32_t minx, miny, maxx, maxy;
+    uint32_t flags;  // bit0: full i16 range needed, bit1: has lines
+    uint32_t pad[3];
+};
+
+// Device counters shared by the kernels (one 256-byte block).
+struct Counters {
+    uint32_t error;        // OR of error bits
+    uint32_t max_w;        // max grid width in tiles
+    uint32_t pad[62];
+};
+
+enum ErrorBits : uint32_t {
+    ERR_FLATTEN = 1u,
+    ERR_BBOX = 2u,       // a run fell outside its path grid (internal invariant)
+    ERR_INPUT = 4u,      // verbs/points/weights inconsistent
+    ERR_REPLAY = 8u,     // a run replay exceeded its step bound (internal invariant)
+    ERR_CAPACITY = 16u,  // a fixed-capacity buffer overflowed (batched path)
+};
+
+// Growable device buffer.
+struct DBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    hipError_t ensure(size_t need);
+    ~DBuf();
+    template <typename T>
+    T* as() const { return reinterpret_cast<T*>(p); }
+};
+
+struct Context {
+    int device = 0;
+    hipStream_t own_stream = nullptr;
+    // scan look-back state
+    DBuf scan_flags, scan_agg, scan_incl, scan_ctl;
+    uint32_t scan_cap_blocks = 0;
+    // pipeline scratch
+    DBuf cmd_scan, cmd_seg, cmd_lines, cmd_line_off;
+    DBuf lines, line_path, line_cnt, line_off;
+    DBuf path_box, path_grid, path_rows;
+    DBuf counters, totals;
+    DBuf keys[2], vals[2], records, rs_counts;
+    DBuf grp_begin, grp_key, grp_hasrun, grp_sign;
+    DBuf row_gbegin, row_gend, row_sign, row_wb, row_path, row_cnt, row_off;
+    // host staging
+    void* host_pinned = nullptr;
+    // timing
+    bool timing = false;
+    std::map<std::string, std::pair<double, int64_t>> timings;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    // batched-path state lives in batched.hip
+    void* batched = nullptr;
+    ~Context();
+};
+
+int run_plain(Context* ctx, const SegInput& in, OutputDesc& out, hipStream_t stream);
+
+// Scan helper bound to a context's look-back state.
+template <typename T>
+ScanState<T> scan_state(Context* ctx) {
+    ScanState<T> s;
+    s.flags = ctx->scan_flags.as<uint32_t>();
+    s.aggregate = ctx->scan_agg.as<T>();
+    s.inclusive = ctx->scan_incl.as<T>();
+    s.ctl = ctx->scan_ctl.as<uint32_t>();
+    return s;
+}
+hipError_t ensure_scan_capacity(Context* ctx, int64_t n);
+
+int map_hip_error(hipError_t e);
+
+}  // namespace ochre

@@ -1,0 +1,1102 @@
+// The plain five-stage pipeline (gfx950):
+//   (a) flatten      k_flatten<false> counts lines per command, a device scan
+//                    allocates slots, k_flatten<true> writes them
+//   (b) binning      k_dda<false>/<true> walk each line's DDA and emit one
+//                    sort entry per tile run and per tile-row crossing,
+//                    keyed (path, tile_y, tile_x); k_rs_* radix-sort them
+//   (c) accumulation k_tiles<true>: one wave per tile row, one lane per tile,
+//                    runs replayed in emission order into LDS accumulators
+//   (d) row scan     the carry (rasterizer.rs:233-250) and the winding of the
+//                    tile-row crossings (rasterizer.rs:252-261)
+//   (e) compaction   tiles and spans written at scanned offsets
+// References: glowcoil/ochre src/path.rs:41-109, src/rasterizer.rs:61-268.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+
+#include "device_common.hpp"
+#include "pipeline.hpp"
+#include "radix_sort.hpp"
+#include "scan.hpp"
+
+namespace ochre {
+
+// ----------------------------------------------------------------- utils
+
+hipError_t DBuf::ensure(size_t need) {
+    if (need <= bytes && p) return hipSuccess;
+    size_t nb = std::max<size_t>(need + need / 4, 4096);
+    if (p) {
+        hipError_t e = hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        if (e != hipSuccess) return e;
+    }
+    hipError_t e = hipMalloc(&p, nb);
+    if (e != hipSuccess) {
+        p = nullptr;
+        return e;
+    }
+    bytes = nb;
+    return hipSuccess;
+}
+
+DBuf::~DBuf() {
+    if (p) (void)hipFree(p);
+}
+
+Context::~Context() {
+    if (host_pinned) (void)hipHostFree(host_pinned);
+    for (auto& e : ev)
+        if (e) (void)hipEventDestroy(e);
+    if (own_stream) (void)hipStreamDestroy(own_stream);
+}
+
+int map_hip_error(hipError_t e) {
+    if (e == hipSuccess) return 0;
+    if (e == hipErrorOutOfMemory) return -2;
+    return -3;
+}
+
+hipError_t ensure_scan_capacity(Context* ctx, int64_t n) {
+    uint32_t blocks = (uint32_t)std::max<int64_t>(1, (n + SCAN_TILE - 1) / SCAN_TILE);
+    if (blocks <= ctx->scan_cap_blocks && ctx->scan_ctl.p) return hipSuccess;
+    uint32_t cap = std::max<uint32_t>(blocks, 1024);
+    hipError_t e;
+    // flags keep their epoch tags; fresh memory must start at zero.
+    DBuf nf;
+    if ((e = nf.ensure(cap * sizeof(uint32_t))) != hipSuccess) return e;
+    if ((e = hipMemset(nf.p, 0, nf.bytes)) != hipSuccess) return e;
+    std::swap(ctx->scan_flags.p, nf.p);
+    std::swap(ctx->scan_flags.bytes, nf.bytes);
+    if ((e = ctx->scan_agg.ensure(cap * 16)) != hipSuccess) return e;
+    if ((e = ctx->scan_incl.ensure(cap * 16)) != hipSuccess) return e;
+    if (!ctx->scan_ctl.p) {
+        if ((e = ctx->scan_ctl.ensure(256)) != hipSuccess) return e;
+        if ((e = hipMemset(ctx->scan_ctl.p, 0, 256)) != hipSuccess) return e;
+    } else {
+        // epochs restart with the new flag array
+        if ((e = hipMemset(ctx->scan_ctl.p, 0, 256)) != hipSuccess) return e;
+    }
+    if ((e = hipDeviceSynchronize()) != hipSuccess) return e;
+    ctx->scan_cap_blocks = cap;
+    return hipSuccess;
+}
+
+template <typename T, typename Op, typename Load, typename Store>
+static hipError_t launch_scan(Context* ctx, int64_t n, Load load, Store store, Op op, T identity, T* total,
+                              hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    uint32_t blocks = (uint32_t)((n + SCAN_TILE - 1) / SCAN_TILE);
+    hipLaunchKernelGGL((k_scan<T, Op, Load, Store>), dim3(blocks), dim3(SCAN_NT), 0, s, load, store, n,
+                       scan_state<T>(ctx), op, identity, total);
+    return hipGetLastError();
+}
+
+template <typename T>
+__global__ void k_fill(T* p, int64_t n, T v) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+template <typename T>
+static void fill(T* p, int64_t n, T v, hipStream_t s) {
+    if (n <= 0) return;
+    int64_t blocks = std::min<int64_t>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_fill<T>, dim3((uint32_t)blocks), dim3(256), 0, s, p, n, v);
+}
+
+static inline uint32_t bits_for(uint64_t n) {  // smallest b with (1 << b) >= n
+    uint32_t b = 0;
+    while (b < 63 && (1ull << b) < n) b++;
+    return b;
+}
+
+// --------------------------------------------------- (a) command prologue
+
+struct CmdAcc {
+    uint32_t pts;
+    uint32_t conics;
+    int32_t move;
+    uint32_t pad;
+};
+struct OpCmd {
+    __device__ CmdAcc operator()(CmdAcc a, CmdAcc b) const {
+        return CmdAcc{a.pts + b.pts, a.conics + b.conics, a.move > b.move ? a.move : b.move, 0u};
+    }
+};
+struct LoadCmd {
+    const uint8_t* verbs;
+    __device__ CmdAcc operator()(int64_t i) const {
+        int v = verbs[i];
+        int np = verb_points(v);
+        return CmdAcc{(uint32_t)(np < 0 ? 0 : np), v == CONIC ? 1u : 0u, v == MOVE ? (int32_t)i : -1, 0u};
+    }
+};
+struct StoreCmd {
+    CmdAcc* out;
+    __device__ void operator()(int64_t i, CmdAcc excl, CmdAcc) const { out[i] = excl; }
+};
+
+// Largest index s in [0, n) with arr[s] <= v (arr non-decreasing, arr[0] <= v).
+__device__ inline int64_t seg_search(const int64_t* arr, int64_t n, int64_t v) {
+    int64_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        int64_t mid = (lo + hi) >> 1;
+        if (arr[mid] <= v) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__device__ inline V2 load_point(const SegInput& in, int64_t g, int64_t s) {
+    V2 p = mk(in.points[2 * g], in.points[2 * g + 1]);
+    if (in.seg_raw && in.seg_raw[s]) return p;
+    return apply(in.seg_xf[s], p);
+}
+
+// Resolved command: its transformed points and the rasterizer state
+// (`last`, `first`) it starts from (rasterizer.rs:145-165).
+struct Cmd {
+    int verb;
+    int32_t path;
+    bool last_in_path;
+    bool bad;
+    V2 pt[3];
+    float w;
+    V2 start, first;
+};
+
+__device__ inline Cmd resolve_cmd(const SegInput& in, const CmdAcc* scan, int64_t i) {
+    Cmd c;
+    int64_t s = seg_search(in.seg_verb_off, in.n_segs, i);
+    int64_t p = in.seg_path ? in.seg_path[s] : s;
+    int64_t fs = in.path_seg_off ? in.path_seg_off[p] : p;
+    int64_t ls = in.path_seg_off ? in.path_seg_off[p + 1] : p + 1;
+    int64_t vb = in.seg_verb_off[fs], ve = in.seg_verb_off[ls];
+    int64_t pb = in.seg_point_off[fs];
+    CmdAcc a = scan[i];
+    int64_t g = a.pts;
+    c.path = (int32_t)p;
+    c.verb = in.verbs[i];
+    c.last_in_path = (i == ve - 1);
+    int np = verb_points(c.verb);
+    c.bad = np < 0;
+    if (i == in.seg_verb_off[s] && (g != in.seg_point_off[s] || (int64_t)a.conics != in.seg_weight_off[s]))
+        c.bad = true;
+    if (np < 0) np = 0;
+    if (g + np > in.n_points) {
+        c.bad = true;
+        np = 0;
+    }
+    for (int k = 0; k < np; k++) c.pt[k] = load_point(in, g + k, s);
+    c.w = 0.0f;
+    if (c.verb == CONIC) {
+        if ((int64_t)a.conics < in.n_weights) c.w = in.weights[a.conics];
+        else c.bad = true;
+    }
+    if (g > pb) {
+        int64_t sp = (g - 1 >= in.seg_point_off[s]) ? s : seg_search(in.seg_point_off, in.n_segs, g - 1);
+        c.start = load_point(in, g - 1, sp);
+    } else {
+        c.start = mk(0.0f, 0.0f);
+    }
+    if (a.move >= vb) {
+        int64_t ms = seg_search(in.seg_verb_off, in.n_segs, a.move);
+        int64_t gm = scan[a.move].pts;
+        if (gm < in.n_points) c.first = load_point(in, gm, ms);
+        else c.bad = true;
+    } else {
+        c.first = mk(0.0f, 0.0f);
+    }
+    return c;
+}
+
+// Enumerates the lines a command feeds to line_to (rasterizer.rs:61-69,
+// 145-157 and the finish-time close, :181-183), in order.
+template <class Emit>
+__device__ inline bool command_lines(const Cmd& c, Emit&& emit) {
+    bool ok = true;
+    V2 last = c.start;
+    V2 first = c.first;
+    switch (c.verb) {
+    case MOVE:
+        if (!veq(last, first)) emit(last, first);
+        last = c.pt[0];
+        first = c.pt[0];
+        break;
+    case LINE:
+        emit(last, c.pt[0]);
+        last = c.pt[0];
+        break;
+    case QUAD: {
+        float dt = quad_dt(last, c.pt[0], c.pt[1]);
+        float t = 0.0f;
+        uint32_t n = 0;
+        V2 prev = last;
+        while (t < 1.0f) {
+            t = fminf(t + dt, 1.0f);
+            V2 q = quad_at(t, last, c.pt[0], c.pt[1]);
+            emit(prev, q);
+            prev = q;
+            if (++n >= FLATTEN_LIMIT) {
+                ok = false;
+                break;
+            }
+        }
+        last = prev;
+        break;
+    }
+    case CUBIC: {
+        float dt = cubic_dt(last, c.pt[0], c.pt[1], c.pt[2]);
+        float t = 0.0f;
+        uint32_t n = 0;
+        V2 prev = last;
+        while (t < 1.0f) {
+            t = fminf(t + dt, 1.0f);
+            V2 q = cubic_at(t, last, c.pt[0], c.pt[1], c.pt[2]);
+            emit(prev, q);
+            prev = q;
+            if (++n >= FLATTEN_LIMIT) {
+                ok = false;
+                break;
+            }
+        }
+        last = prev;
+        break;
+    }
+    case CONIC: {
+        V2 prev = last;
+        ok = conic_walk(last, c.pt[0], c.pt[1], c.w, [&](V2 q) {
+            emit(prev, q);
+            prev = q;
+        });
+        last = prev;
+        break;
+    }
+    default:  // CLOSE: a no-op in the rasterizer (rasterizer.rs:154)
+        break;
+    }
+    if (c.last_in_path && !veq(last, first)) emit(last, first);
+    return ok;
+}
+
+constexpr float HUGE_COORD = 30000.0f;
+enum BoxFlags : uint32_t { BOX_FULL = 1u, BOX_LINES = 2u };
+
+struct BoxAcc {
+    float minx = INFINITY, miny = INFINITY, maxx = -INFINITY, maxy = -INFINITY;
+    bool full = false;
+    __device__ void add(V2 p) {
+        if (!(fabsf(p.x) <= HUGE_COORD) || !(fabsf(p.y) <= HUGE_COORD)) {
+            full = true;
+            return;
+        }
+        minx = fminf(minx, p.x);
+        miny = fminf(miny, p.y);
+        maxx = fmaxf(maxx, p.x);
+        maxy = fmaxf(maxy, p.y);
+    }
+};
+
+struct FlattenArgs {
+    SegInput in;
+    const CmdAcc* scan;
+    uint32_t* cmd_lines;
+    const uint32_t* cmd_line_off;
+    float4* lines;
+    uint32_t* line_path;
+    PathBox* box;
+    Counters* cnt;
+};
+
+template <bool EMIT>
+__global__ __launch_bounds__(256) void k_flatten(FlattenArgs a) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= a.in.n_verbs) return;
+    Cmd c = resolve_cmd(a.in, a.scan, i);
+    if (c.bad) {
+        atomicOr(&a.cnt->error, (uint32_t)ERR_INPUT);
+        if (!EMIT) a.cmd_lines[i] = 0;
+        return;
+    }
+    if (!EMIT) {
+        uint32_t n = 0;
+        BoxAcc box;
+        bool ok = command_lines(c, [&](V2 p0, V2 p1) {
+            n++;
+            box.add(p0);
+            box.add(p1);
+        });
+        if (!ok) atomicOr(&a.cnt->error, (uint32_t)ERR_FLATTEN);
+        a.cmd_lines[i] = n;
+        if (n > 0) {
+            PathBox* b = &a.box[c.path];
+            if (box.full) atomicOr(&b->flags, (uint32_t)(BOX_FULL | BOX_LINES));
+            else atomicOr(&b->flags, (uint32_t)BOX_LINES);
+            if (box.minx <= box.maxx) {
+                atomicMin(&b->minx, (int32_t)floorf(box.minx));
+                atomicMin(&b->miny, (int32_t)floorf(box.miny));
+                atomicMax(&b->maxx, (int32_t)floorf(box.maxx));
+                atomicMax(&b->maxy, (int32_t)floorf(box.maxy));
+            }
+        }
+    } else {
+        uint32_t o = a.cmd_line_off[i];
+        command_lines(c, [&](V2 p0, V2 p1) {
+            a.lines[o] = make_float4(p0.x, p0.y, p1.x, p1.y);
+            a.line_path[o] = (uint32_t)c.path;
+            o++;
+        });
+    }
+}
+
+// Per-path tile grid from the bounding box of all line end points, widened
+// by the DDA's worst-case overshoot: the crossing parameters are summed
+// sequentially (rasterizer.rs:110,114), so after k steps they may be off by
+// k * 2^-24, i.e. up to span^2 * 2^-24 pixels, plus one pixel of floor().
+__global__ void k_path_grid(const PathBox* box, PathGrid* grid, uint32_t* rows, int64_t n_paths, Counters* cnt) {
+    int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (p >= n_paths) return;
+    PathBox b = box[p];
+    PathGrid g;
+    g.tx0 = g.ty0 = g.w = g.h = 0;
+    if (b.flags & BOX_LINES) {
+        if (b.flags & BOX_FULL) {
+            g.tx0 = g.ty0 = -4096;
+            g.w = g.h = 8192;
+        } else {
+            int64_t span = std::max<int64_t>((int64_t)b.maxx - b.minx, (int64_t)b.maxy - b.miny) + 2;
+            int64_t margin = 2 + (int64_t)ceil((double)span * (double)span * 1.2e-7);
+            int64_t px0 = std::max<int64_t>((int64_t)b.minx - margin, -32768);
+            int64_t py0 = std::max<int64_t>((int64_t)b.miny - margin, -32768);
+            int64_t px1 = std::min<int64_t>((int64_t)b.maxx + margin, 32767);
+            int64_t py1 = std::min<int64_t>((int64_t)b.maxy + margin, 32767);
+            g.tx0 = (int32_t)(px0 >> 3);
+            g.ty0 = (int32_t)(py0 >> 3);
+            g.w = (int32_t)((px1 >> 3) - g.tx0 + 1);
+            g.h = (int32_t)((py1 >> 3) - g.ty0 + 1);
+        }
+        atomicMax(&cnt->max_w, (uint32_t)g.w);
+    }
+    g.row_base = 0;
+    grid[p] = g;
+    rows[p] = (uint32_t)g.h + 1;  // + one row that carries the empty-path tile
+}
+
+struct LoadU32 {
+    const uint32_t* p;
+    __device__ uint32_t operator()(int64_t i) const { return p[i]; }
+};
+struct StoreU32 {
+    uint32_t* p;
+    __device__ void operator()(int64_t i, uint32_t excl, uint32_t) const { p[i] = excl; }
+};
+struct StoreRowBase {
+    PathGrid* g;
+    __device__ void operator()(int64_t i, uint32_t excl, uint32_t) const { g[i].row_base = excl; }
+};
+
+// ------------------------------------------------------ (b) DDA binning
+//
+// Walks Rasterizer::line_to (rasterizer.rs:72-140) for one line and reports
+// tile runs (maximal runs of consecutive increments in one tile) and
+// tile-row crossings (TileIncrement, :123-131).  A run record holds the DDA
+// state one step before the run's first increment (or "line start"), which
+// is enough to replay the run bit-exactly later.
+struct RunRec {
+    uint32_t line;  // bit 31: run starts at the line's first increment
+    uint32_t xy;    // x | y << 16 (i16) of the state before the entering step
+    float row_t1, col_t1;
+};
+
+// Steps per line before the walk gives up: a line spanning the whole i16
+// pixel range needs ~2^17; more only arise from coordinates so large that
+// the reference would exhaust memory pushing increments.
+constexpr uint32_t WALK_LIMIT = 1u << 20;
+
+template <class Sink>
+__device__ inline void walk_line(V2 last, V2 point, Sink& sink) {
+    if (veq(point, last)) return;
+    Dda d;
+    dda_init(d, last, point);
+    int16_t ty_prev = div8(d.y);
+    int16_t cur_tx = div8(d.x), cur_ty = ty_prev;
+    sink.run(cur_tx, cur_ty, true, d);
+    for (uint32_t steps = 0;; steps++) {
+        if (steps >= WALK_LIMIT) {
+            sink.limit();
+            break;
+        }
+        Dda before = d;
+        bool term = dda_step(d);
+        if (term) {
+            d.x = f2i16(floorf(point.x));
+            d.y = f2i16(floorf(point.y));
+        }
+        int16_t ty = div8(d.y);
+        if (ty != ty_prev) {
+            sink.crossing(div8(d.x), ty_prev < ty ? ty_prev : ty, (int8_t)(int16_t)(ty - ty_prev));
+            ty_prev = ty;
+        }
+        if (term) break;
+        int16_t tx = div8(d.x);
+        if (tx != cur_tx || ty != cur_ty) {
+            cur_tx = tx;
+            cur_ty = ty;
+            sink.run(tx, ty, false, before);
+        }
+    }
+}
+
+struct CountSink {
+    uint32_t runs = 0, entries = 0;
+    __device__ void run(int16_t, int16_t, bool, const Dda&) {
+        runs++;
+        entries++;
+    }
+    __device__ void crossing(int16_t, int16_t, int8_t) { entries++; }
+    Counters* cnt;
+    __device__ void limit() { atomicOr(&cnt->error, (uint32_t)ERR_FLATTEN); }
+};
+
+struct EmitSink {
+    uint32_t line;
+    uint32_t entry, run_idx;
+    PathGrid g;
+    uint32_t xbits;
+    uint32_t* keys;
+    uint32_t* vals;
+    RunRec* recs;
+    Counters* cnt;
+    __device__ bool key_of(int16_t tx, int16_t ty, uint32_t* k) {
+        uint32_t lx = (uint32_t)((int32_t)tx - g.tx0);
+        uint32_t ly = (uint32_t)((int32_t)ty - g.ty0);
+        if (lx >= (uint32_t)g.w || ly >= (uint32_t)g.h) {
+            atomicOr(&cnt->error, (uint32_t)ERR_BBOX);
+            return false;
+        }
+        *k = ((g.row_base + ly) << xbits) | lx;
+        return true;
+    }
+    __device__ void run(int16_t tx, int16_t ty, bool start, const Dda& d) {
+        uint32_t k = 0;
+        key_of(tx, ty, &k);
+        keys[entry] = k;
+        vals[entry] = run_idx;
+        RunRec r;
+        r.line = line | (start ? 0x80000000u : 0u);
+        r.xy = (uint32_t)(uint16_t)d.x | ((uint32_t)(uint16_t)d.y << 16);
+        r.row_t1 = d.row_t1;
+        r.col_t1 = d.col_t1;
+        recs[run_idx] = r;
+        entry++;
+        run_idx++;
+    }
+    __device__ void limit() {}
+    __device__ void crossing(int16_t tx, int16_t ty, int8_t sign) {
+        uint32_t k = 0;
+        key_of(tx, ty, &k);
+        keys[entry] = k;
+        vals[entry] = 0x80000000u | (uint32_t)(uint8_t)sign;
+        entry++;
+    }
+};
+
+struct DdaArgs {
+    const float4* lines;
+    const uint32_t* line_path;
+    int64_t n_lines;
+    uint64_t* line_cnt;        // count pass: (runs << 32) | entries; emit pass: exclusive offsets
+    const PathGrid* grid;
+    uint32_t xbits;
+    uint32_t* keys;
+    uint32_t* vals;
+    RunRec* recs;
+    Counters* cnt;
+};
+
+template <bool EMIT>
+__global__ __launch_bounds__(256) void k_dda(DdaArgs a) {
+    int64_t l = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (l >= a.n_lines) return;
+    float4 L = a.lines[l];
+    if (!EMIT) {
+        CountSink s;
+        s.cnt = a.cnt;
+        walk_line(mk(L.x, L.y), mk(L.z, L.w), s);
+        a.line_cnt[l] = ((uint64_t)s.runs << 32) | s.entries;
+    } else {
+        uint64_t off = a.line_cnt[l];
+        EmitSink s;
+        s.line = (uint32_t)l;
+        s.entry = (uint32_t)off;
+        s.run_idx = (uint32_t)(off >> 32);
+        s.g = a.grid[a.line_path[l]];
+        s.xbits = a.xbits;
+        s.keys = a.keys;
+        s.vals = a.vals;
+        s.recs = a.recs;
+        s.cnt = a.cnt;
+        walk_line(mk(L.x, L.y), mk(L.z, L.w), s);
+    }
+}
+
+struct LoadU64 {
+    const uint64_t* p;
+    __device__ uint64_t operator()(int64_t i) const { return p[i]; }
+};
+struct StoreU64 {
+    uint64_t* p;
+    __device__ void operator()(int64_t i, uint64_t excl, uint64_t) const { p[i] = excl; }
+};
+
+// ------------------------------------------------ groups and tile rows
+
+struct LoadGroupStart {
+    const uint32_t* keys;
+    __device__ uint32_t operator()(int64_t i) const { return (i == 0 || keys[i] != keys[i - 1]) ? 1u : 0u; }
+};
+struct StoreGroup {
+    const uint32_t* keys;
+    const uint32_t* vals;
+    uint32_t* gbegin;
+    uint32_t* gkey;
+    uint32_t* ghasrun;
+    int32_t* gsign;
+    int64_t n;
+    __device__ void operator()(int64_t i, uint32_t excl, uint32_t flag) const {
+        uint32_t gid = excl + flag - 1;
+        if (flag) {
+            gbegin[gid] = (uint32_t)i;
+            gkey[gid] = keys[i];
+        }
+        if (i == n - 1) gbegin[gid + 1] = (uint32_t)n;
+        uint32_t v = vals[i];
+        if (v >> 31) {
+            int s = (int8_t)(v & 0xffu);
+            if (s) atomicAdd(&gsign[gid], s);
+        } else {
+            ghasrun[gid] = 1u;
+        }
+    }
+};
+
+struct RowArgs {
+    const uint32_t* gkey;
+    const int32_t* gsign;
+    const uint32_t* n_groups;
+    uint32_t xbits;
+    uint32_t* row_gbegin;
+    uint32_t* row_gend;
+    int32_t* row_sign;
+};
+
+__global__ void k_row_table(RowArgs a, int64_t cap) {
+    const uint32_t G = *a.n_groups;
+    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < G && g < cap;
+         g += (int64_t)gridDim.x * blockDim.x) {
+        uint32_t row = a.gkey[g] >> a.xbits;
+        if (g == 0 || (a.gkey[g - 1] >> a.xbits) != row) a.row_gbegin[row] = (uint32_t)g;
+        if (g == G - 1 || (a.gkey[g + 1] >> a.xbits) != row) a.row_gend[row] = (uint32_t)g + 1;
+        int s = a.gsign[g];
+        if (s) atomicAdd(&a.row_sign[row], s);
+    }
+}
+
+// Per path: winding entering each tile row (the running sum of all earlier
+// crossings, rasterizer.rs:253-260), the row -> path map, and the
+// empty-path tile (rasterizer.rs:193-198: no increments still emits one
+// all-zero tile at (0, 0)).
+__global__ void k_path_rows(const PathGrid* grid, int64_t n_paths, const uint32_t* row_gbegin,
+                            const uint32_t* row_gend, const int32_t* row_sign, int32_t* row_wb,
+                            uint32_t* row_path, uint64_t* row_cnt) {
+    int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (p >= n_paths) return;
+    PathGrid g = grid[p];
+    int32_t running = 0;
+    bool nonempty = false;
+    for (int r = 0; r < g.h; r++) {
+        uint32_t row = g.row_base + r;
+        row_wb[row] = running;
+        running += row_sign[row];
+        nonempty |= row_gend[row] > row_gbegin[row];
+        row_path[row] = (uint32_t)p;
+    }
+    uint32_t filler = g.row_base + g.h;
+    row_path[filler] = (uint32_t)p;
+    row_cnt[filler] = nonempty ? 0ull : (1ull << 32);
+}
+
+// ----------------------------------------- (c)(d)(e) tile rows, one wave each
+
+struct TileArgs {
+    const PathGrid* grid;
+    const uint32_t* row_path;
+    const uint32_t* row_gbegin;
+    const uint32_t* row_gend;
+    const int32_t* row_wb;
+    int64_t n_rows;
+    const uint32_t* gbegin;
+    const uint32_t* gkey;
+    const uint32_t* ghasrun;
+    const int32_t* gsign;
+    uint32_t xbits;
+    const uint32_t* vals;     // sorted entry values
+    const RunRec* recs;
+    const float4* lines;
+    uint64_t* row_cnt;        // count pass output
+    const uint64_t* row_off;  // emit pass input: (tiles << 32) | spans exclusive offsets
+    int16_t* tile_xy;
+    uint8_t* tile_alpha;
+    int16_t* span_xy;
+    uint16_t* span_w;
+    Counters* cnt;
+};
+
+constexpr int REPLAY_LIMIT = 64;  // a tile run has at most 15 increments
+
+// Replays one tile run into this lane's accumulator column.
+__device__ inline void replay_run(const RunRec& r, const float4* lines, int16_t tx, int16_t ty, float* acc_a,
+                                  float* acc_h, int lane, Counters* cnt) {
+    float4 L = lines[r.line & 0x7fffffffu];
+    Dda d;
+    V2 last = mk(L.x, L.y), point = mk(L.z, L.w);
+    if (r.line >> 31) {
+        dda_init(d, last, point);
+    } else {
+        dda_consts(d, last, point);
+        d.x = (int16_t)(r.xy & 0xffffu);
+        d.y = (int16_t)(r.xy >> 16);
+        d.row_t1 = r.row_t1;
+        d.col_t1 = r.col_t1;
+        d.t0 = 0.0f;
+        dda_step(d);
+    }
+    for (int k = 0; k < REPLAY_LIMIT; k++) {
+        float area, height;
+        dda_increment(d, area, height);
+        int pix = ((d.y & 7) << 3) | (d.x & 7);
+        acc_a[pix * WAVE + lane] += area;
+        acc_h[pix * WAVE + lane] += height;
+        if (dda_step(d)) return;
+        if (div8(d.x) != tx || div8(d.y) != ty) return;
+    }
+    atomicOr(&cnt->error, (uint32_t)ERR_REPLAY);
+}
+
+template <bool EMIT>
+__global__ __launch_bounds__(WAVE) void k_tiles(TileArgs a) {
+    __shared__ float acc_a[EMIT ? 64 * WAVE : 1];
+    __shared__ float acc_h[EMIT ? 64 * WAVE : 1];
+    __shared__ float cin[EMIT ? WAVE * 8 : 1];
+    const int lane = threadIdx.x;
+    const uint64_t lt = lanemask_lt();
+    const uint64_t le = lane == 63 ? ~0ull : ((2ull << lane) - 1);
+    const uint32_t xmask = (1u << a.xbits) - 1;
+    for (int64_t row = blockIdx.x; row < a.n_rows; row += gridDim.x) {
+        const uint32_t p = a.row_path[row];
+        const PathGrid G = a.grid[p];
+        const int64_t local = row - (int64_t)G.row_base;
+        if (local >= G.h) continue;  // the empty-path row
+        const uint32_t gb = a.row_gbegin[row], ge = a.row_gend[row];
+        if (gb >= ge) {
+            if (!EMIT && lane == 0) a.row_cnt[row] = 0;
+            continue;
+        }
+        const int32_t ty = G.ty0 + (int32_t)local;
+        int32_t winding = a.row_wb[row];
+        uint32_t n_tiles = 0, n_spans = 0;
+        bool pend = false;
+        int32_t pend_tx = 0, pend_w = 0;
+        float carry = 0.0f;
+        uint64_t off = EMIT ? a.row_off[row] : 0;
+        const uint32_t tile_base = (uint32_t)(off >> 32), span_base = (uint32_t)off;
+        for (uint32_t c = gb; c < ge; c += WAVE) {
+            const uint32_t g = c + lane;
+            const bool valid = g < ge;
+            const uint32_t key = valid ? a.gkey[g] : 0u;
+            const int32_t tx = G.tx0 + (int32_t)(key & xmask);
+            const bool hr = valid && a.ghasrun[g];
+            const int32_t sg = valid ? a.gsign[g] : 0;
+            const int32_t incl = wave_incl_scan(sg, OpAdd());
+            const int32_t wend = winding + incl;  // winding after this key's crossings
+            winding += __shfl(incl, WAVE - 1, WAVE);
+            const uint64_t tiles = __ballot(hr);
+            // spans between consecutive tiles of this chunk (rasterizer.rs:252-264)
+            const uint64_t above = tiles & ~le;
+            const int nb = above ? (__ffsll((long long)above) - 1) : lane;
+            const int32_t btx = __shfl(tx, nb, WAVE);
+            const bool span_here = hr && above && btx > tx + 1 && wend != 0;
+            // span closing the previous chunk's last tile
+            const int ft = tiles ? (__ffsll((long long)tiles) - 1) : 0;
+            const int32_t ftx = __shfl(tx, ft, WAVE);
+            bool pend_span = false;
+            if (pend && tiles) {
+                pend_span = ftx > pend_tx + 1 && pend_w != 0;
+                pend = false;
+            }
+            const uint64_t spans = __ballot(span_here);
+            if (EMIT) {
+                if (pend_span && lane == 0) {
+                    uint32_t si = span_base + n_spans;
+                    a.span_xy[2 * si] = (int16_t)((pend_tx + 1) * TILE);
+                    a.span_xy[2 * si + 1] = (int16_t)(ty * TILE);
+                    a.span_w[si] = (uint16_t)((uint16_t)(ftx - pend_tx - 1) * (uint16_t)TILE);
+                }
+                if (span_here) {
+                    uint32_t si = span_base + n_spans + (pend_span ? 1u : 0u) + (uint32_t)__popcll(spans & lt);
+                    a.span_xy[2 * si] = (int16_t)((tx + 1) * TILE);
+                    a.span_xy[2 * si + 1] = (int16_t)(ty * TILE);
+                    a.span_w[si] = (uint16_t)((uint16_t)(btx - tx - 1) * (uint16_t)TILE);
+                }
+            }
+            n_spans += (pend_span ? 1u : 0u) + (uint32_t)__popcll(spans);
+            if (tiles) {
+                const int lt_lane = 63 - __clzll((long long)tiles);
+                pend = true;
+                pend_tx = __shfl(tx, lt_lane, WAVE);
+                pend_w = __shfl(wend, lt_lane, WAVE);
+            }
+            if (EMIT) {
+                // (c) accumulation, one lane per tile, runs in emission order
+                for (int pix = 0; pix < 64; pix++) {
+                    acc_a[pix * WAVE + lane] = 0.0f;
+                    acc_h[pix * WAVE + lane] = 0.0f;
+                }
+                if (hr) {
+                    const uint32_t e0 = a.gbegin[g], e1 = a.gbegin[g + 1];
+                    const int16_t ttx = (int16_t)tx, tty = (int16_t)ty;
+                    for (uint32_t e = e0; e < e1; e++) {
+                        uint32_t v = a.vals[e];
+                        if (v >> 31) continue;
+                        RunRec r = a.recs[v];
+                        replay_run(r, a.lines, ttx, tty, acc_a, acc_h, lane, a.cnt);
+                    }
+                }
+                __syncthreads();
+                // (d) carry along the tile row, one lane per pixel row, sequential
+                if (lane < 8) {
+                    uint64_t tm = tiles;
+                    while (tm) {
+                        int t = __ffsll((long long)tm) - 1;
+                        tm &= tm - 1;
+                        cin[t * 8 + lane] = carry;
+#pragma unroll
+                        for (int x = 0; x < 8; x++) carry += acc_h[(lane * 8 + x) * WAVE + t];
+                    }
+                }
+                __syncthreads();
+                // (e) quantise and write (rasterizer.rs:231-241)
+                if (hr) {
+                    uint32_t ti = tile_base + n_tiles + (uint32_t)__popcll(tiles & lt);
+                    uint32_t words[16];
+#pragma unroll
+                    for (int y = 0; y < 8; y++) {
+                        float accum = cin[lane * 8 + y];
+                        uint32_t w0 = 0, w1 = 0;
+#pragma unroll
+                        for (int x = 0; x < 8; x++) {
+                            int pix = y * 8 + x;
+                            float v = fminf(fabsf(accum + acc_a[pix * WAVE + lane]) * 256.0f, 255.0f);
+                            uint32_t b = (uint32_t)v;
+                            if (x < 4) w0 |= b << (8 * x);
+                            else w1 |= b << (8 * (x - 4));
+                            accum += acc_h[pix * WAVE + lane];
+                        }
+                        words[2 * y] = w0;
+                        words[2 * y + 1] = w1;
+                    }
+                    uint4* dst = reinterpret_cast<uint4*>(a.tile_alpha + (size_t)ti * 64);
+                    dst[0] = make_uint4(words[0], words[1], words[2], words[3]);
+                    dst[1] = make_uint4(words[4], words[5], words[6], words[7]);
+                    dst[2] = make_uint4(words[8], words[9], words[10], words[11]);
+                    dst[3] = make_uint4(words[12], words[13], words[14], words[15]);
+                    a.tile_xy[2 * ti] = (int16_t)(tx * TILE);
+                    a.tile_xy[2 * ti + 1] = (int16_t)(ty * TILE);
+                }
+                __syncthreads();
+            }
+            n_tiles += (uint32_t)__popcll(tiles);
+        }
+        if (!EMIT && lane == 0) a.row_cnt[row] = ((uint64_t)n_tiles << 32) | n_spans;
+    }
+}
+
+// Per-path output ranges and the empty-path tile.
+__global__ void k_path_out(const PathGrid* grid, int64_t n_paths, const uint64_t* row_off, const uint64_t* total,
+                           int64_t* tstart, int32_t* tcount, int64_t* sstart, int32_t* scount, int16_t* tile_xy,
+                           uint8_t* tile_alpha) {
+    int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (p >= n_paths) return;
+    PathGrid g = grid[p];
+    uint64_t b = row_off[g.row_base];
+    uint64_t e = (p + 1 < n_paths) ? row_off[grid[p + 1].row_base] : *total;
+    uint64_t fb = row_off[g.row_base + g.h];
+    tstart[p] = (int64_t)(b >> 32);
+    tcount[p] = (int32_t)((e >> 32) - (b >> 32));
+    sstart[p] = (int64_t)(uint32_t)b;
+    scount[p] = (int32_t)((uint32_t)e - (uint32_t)b);
+    if ((e >> 32) - (fb >> 32) == 1) {  // the empty-path tile
+        uint64_t ti = fb >> 32;
+        tile_xy[2 * ti] = 0;
+        tile_xy[2 * ti + 1] = 0;
+        uint4* dst = reinterpret_cast<uint4*>(tile_alpha + ti * 64);
+        for (int k = 0; k < 4; k++) dst[k] = make_uint4(0, 0, 0, 0);
+    }
+}
+
+// ------------------------------------------------------------- host driver
+
+#define HIPCHK(x)                                   \
+    do {                                            \
+        hipError_t e_ = (x);                        \
+        if (e_ != hipSuccess) return map_hip_error(e_); \
+    } while (0)
+
+static inline uint32_t grid_for(int64_t n, int bs) { return (uint32_t)((n + bs - 1) / bs); }
+
+struct HostTotals {
+    uint32_t lines;
+    uint32_t error;
+    uint32_t max_w;
+    uint32_t rows;
+    uint64_t dda;     // (runs << 32) | entries
+    uint32_t groups;
+    uint32_t pad;
+    uint64_t out;     // (tiles << 32) | spans
+};
+
+int run_plain(Context* ctx, const SegInput& in, OutputDesc& out, hipStream_t s) {
+    const int64_t V = in.n_verbs;
+    const int64_t NP = in.n_paths;
+    if (NP <= 0) {
+        out.n_tiles = out.n_spans = 0;
+        return 0;
+    }
+    if (V >= (1ll << 31) || in.n_points >= (1ll << 32)) return -7;
+    HIPCHK(ctx->counters.ensure(256));
+    HIPCHK(ctx->totals.ensure(256));
+    if (!ctx->host_pinned) HIPCHK(hipHostMalloc(&ctx->host_pinned, 4096));
+    Counters* cnt = ctx->counters.as<Counters>();
+    uint64_t* tot = ctx->totals.as<uint64_t>();  // [0] lines, [1] rows, [2] dda, [3] groups, [4] out
+    HostTotals* ht = reinterpret_cast<HostTotals*>(ctx->host_pinned);
+    fill<uint32_t>(reinterpret_cast<uint32_t*>(cnt), 64, 0u, s);
+    fill<uint64_t>(tot, 32, 0ull, s);
+
+    HIPCHK(ensure_scan_capacity(ctx, std::max<int64_t>({V, NP, 1})));
+
+    // ---- (a) flatten
+    HIPCHK(ctx->cmd_scan.ensure(std::max<int64_t>(V, 1) * sizeof(CmdAcc)));
+    HIPCHK(ctx->cmd_lines.ensure(std::max<int64_t>(V, 1) * 4));
+    HIPCHK(ctx->cmd_line_off.ensure(std::max<int64_t>(V, 1) * 4));
+    HIPCHK(ctx->path_box.ensure(NP * sizeof(PathBox)));
+    HIPCHK(ctx->path_grid.ensure(NP * sizeof(PathGrid)));
+    HIPCHK(ctx->path_rows.ensure(NP * 4));
+    CmdAcc* cscan = ctx->cmd_scan.as<CmdAcc>();
+    PathBox* box = ctx->path_box.as<PathBox>();
+    {
+        PathBox init;
+        init.minx = init.miny = INT32_MAX;
+        init.maxx = init.maxy = INT32_MIN;
+        init.flags = 0;
+        init.pad[0] = init.pad[1] = init.pad[2] = 0;
+        fill<PathBox>(box, NP, init, s);
+    }
+    HIPCHK(launch_scan<CmdAcc>(ctx, V, LoadCmd{in.verbs}, StoreCmd{cscan}, OpCmd(), CmdAcc{0, 0, -1, 0},
+                               (CmdAcc*)nullptr, s));
+    FlattenArgs fa;
+    fa.in = in;
+    fa.scan = cscan;
+    fa.cmd_lines = ctx->cmd_lines.as<uint32_t>();
+    fa.cmd_line_off = ctx->cmd_line_off.as<uint32_t>();
+    fa.box = box;
+    fa.cnt = cnt;
+    fa.lines = nullptr;
+    fa.line_path = nullptr;
+    if (V > 0) {
+        hipLaunchKernelGGL(k_flatten<false>, dim3(grid_for(V, 256)), dim3(256), 0, s, fa);
+        HIPCHK(launch_scan<uint32_t>(ctx, V, LoadU32{fa.cmd_lines}, StoreU32{ctx->cmd_line_off.as<uint32_t>()},
+                                     OpAdd(), 0u, reinterpret_cast<uint32_t*>(&tot[0]), s));
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(&ht->lines, &tot[0], 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(&ht->error, &cnt->error, 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (ht->error & ERR_INPUT) return -1;
+    if (ht->error & ERR_FLATTEN) return -5;
+    const int64_t L = ht->lines;
+
+    HIPCHK(ctx->lines.ensure(std::max<int64_t>(L, 1) * 16));
+    HIPCHK(ctx->line_path.ensure(std::max<int64_t>(L, 1) * 4));
+    HIPCHK(ctx->line_cnt.ensure(std::max<int64_t>(L, 1) * 8));
+    HIPCHK(ensure_scan_capacity(ctx, std::max<int64_t>(L, 1)));
+    fa.lines = ctx->lines.as<float4>();
+    fa.line_path = ctx->line_path.as<uint32_t>();
+    if (V > 0) hipLaunchKernelGGL(k_flatten<true>, dim3(grid_for(V, 256)), dim3(256), 0, s, fa);
+    PathGrid* grid = ctx->path_grid.as<PathGrid>();
+    hipLaunchKernelGGL(k_path_grid, dim3(grid_for(NP, 256)), dim3(256), 0, s, box, grid,
+                       ctx->path_rows.as<uint32_t>(), NP, cnt);
+    HIPCHK(launch_scan<uint32_t>(ctx, NP, LoadU32{ctx->path_rows.as<uint32_t>()}, StoreRowBase{grid}, OpAdd(), 0u,
+                                 reinterpret_cast<uint32_t*>(&tot[1]), s));
+    DdaArgs da;
+    da.lines = fa.lines;
+    da.line_path = fa.line_path;
+    da.n_lines = L;
+    da.line_cnt = ctx->line_cnt.as<uint64_t>();
+    da.grid = grid;
+    da.cnt = cnt;
+    da.xbits = 0;
+    da.keys = da.vals = nullptr;
+    da.recs = nullptr;
+    if (L > 0) {
+        hipLaunchKernelGGL(k_dda<false>, dim3(grid_for(L, 256)), dim3(256), 0, s, da);
+        HIPCHK(launch_scan<uint64_t>(ctx, L, LoadU64{da.line_cnt}, StoreU64{da.line_cnt}, OpAdd(), 0ull, &tot[2], s));
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(&ht->rows, &tot[1], 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(&ht->dda, &tot[2], 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(&ht->max_w, &cnt->max_w, 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(&ht->error, &cnt->error, 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (ht->error & ERR_FLATTEN) return -5;
+    const int64_t R = (int64_t)(uint32_t)ht->dda;
+    const int64_t RUNS = (int64_t)(ht->dda >> 32);
+    const int64_t ROWS = ht->rows;
+    const uint32_t xbits = std::max<uint32_t>(1, bits_for(std::max<uint32_t>(ht->max_w, 1)));
+    const uint32_t rbits = bits_for((uint64_t)ROWS);
+    if (xbits + rbits > 32 || L >= (1ll << 31) || RUNS >= (1ll << 31)) return -7;
+    const uint32_t kbits = xbits + rbits;
+
+    // ---- (b) binning + sort
+    HIPCHK(ctx->keys[0].ensure(std::max<int64_t>(R, 1) * 4));
+    HIPCHK(ctx->keys[1].ensure(std::max<int64_t>(R, 1) * 4));
+    HIPCHK(ctx->vals[0].ensure(std::max<int64_t>(R, 1) * 4));
+    HIPCHK(ctx->vals[1].ensure(std::max<int64_t>(R, 1) * 4));
+    HIPCHK(ctx->records.ensure(std::max<int64_t>(RUNS, 1) * sizeof(RunRec)));
+    const uint32_t rs_blocks = (uint32_t)std::max<int64_t>(1, (R + RS_TILE - 1) / RS_TILE);
+    HIPCHK(ctx->rs_counts.ensure((size_t)rs_blocks * 256 * 4));
+    HIPCHK(ensure_scan_capacity(ctx, std::max<int64_t>({R, (int64_t)rs_blocks * 256, ROWS, 1})));
+    da.xbits = xbits;
+    da.keys = ctx->keys[0].as<uint32_t>();
+    da.vals = ctx->vals[0].as<uint32_t>();
+    da.recs = ctx->records.as<RunRec>();
+    if (L > 0) hipLaunchKernelGGL(k_dda<true>, dim3(grid_for(L, 256)), dim3(256), 0, s, da);
+    int cur = 0;
+    const int passes = (int)((kbits + 7) / 8);
+    uint32_t* rsc = ctx->rs_counts.as<uint32_t>();
+    for (int pass = 0; pass < passes && R > 0; pass++) {
+        int shift = pass * 8;
+        hipLaunchKernelGGL(k_rs_count, dim3(rs_blocks), dim3(RS_NT), 0, s, ctx->keys[cur].as<uint32_t>(), R, shift,
+                           rsc, rs_blocks);
+        HIPCHK(launch_scan<uint32_t>(ctx, (int64_t)rs_blocks * 256, LoadU32{rsc}, StoreU32{rsc}, OpAdd(), 0u,
+                                     (uint32_t*)nullptr, s));
+        // values are the entry payloads (run index, or crossing sign with bit 31)
+        hipLaunchKernelGGL(k_rs_scatter, dim3(rs_blocks), dim3(RS_NT), 0, s, ctx->keys[cur].as<uint32_t>(),
+                           ctx->vals[cur].as<uint32_t>(), R, shift, rsc, rs_blocks,
+                           ctx->keys[cur ^ 1].as<uint32_t>(), ctx->vals[cur ^ 1].as<uint32_t>());
+        cur ^= 1;
+    }
+    const uint32_t* skeys = ctx->keys[cur].as<uint32_t>();
+    const uint32_t* svals = ctx->vals[cur].as<uint32_t>();
+
+    // ---- groups (one per distinct key) and tile rows
+    HIPCHK(ctx->grp_begin.ensure((std::max<int64_t>(R, 1) + 1) * 4));
+    HIPCHK(ctx->grp_key.ensure(std::max<int64_t>(R, 1) * 4));
+    HIPCHK(ctx->grp_hasrun.ensure(std::max<int64_t>(R, 1) * 4));
+    HIPCHK(ctx->grp_sign.ensure(std::max<int64_t>(R, 1) * 4));
+    fill<uint32_t>(ctx->grp_hasrun.as<uint32_t>(), R, 0u, s);
+    fill<int32_t>(ctx->grp_sign.as<int32_t>(), R, 0, s);
+    if (R > 0) {
+        HIPCHK(launch_scan<uint32_t>(ctx, R, LoadGroupStart{skeys},
+                                     StoreGroup{skeys, svals, ctx->grp_begin.as<uint32_t>(),
+                                                ctx->grp_key.as<uint32_t>(), ctx->grp_hasrun.as<uint32_t>(),
+                                                ctx->grp_sign.as<int32_t>(), R},
+                                     OpAdd(), 0u, reinterpret_cast<uint32_t*>(&tot[3]), s));
+    }
+    HIPCHK(ctx->row_gbegin.ensure(ROWS * 4));
+    HIPCHK(ctx->row_gend.ensure(ROWS * 4));
+    HIPCHK(ctx->row_sign.ensure(ROWS * 4));
+    HIPCHK(ctx->row_wb.ensure(ROWS * 4));
+    HIPCHK(ctx->row_path.ensure(ROWS * 4));
+    HIPCHK(ctx->row_cnt.ensure(ROWS * 8));
+    HIPCHK(ctx->row_off.ensure(ROWS * 8));
+    fill<uint32_t>(ctx->row_gbegin.as<uint32_t>(), ROWS, 0u, s);
+    fill<uint32_t>(ctx->row_gend.as<uint32_t>(), ROWS, 0u, s);
+    fill<int32_t>(ctx->row_sign.as<int32_t>(), ROWS, 0, s);
+    RowArgs ra;
+    ra.gkey = ctx->grp_key.as<uint32_t>();
+    ra.gsign = ctx->grp_sign.as<int32_t>();
+    ra.n_groups = reinterpret_cast<const uint32_t*>(&tot[3]);
+    ra.xbits = xbits;
+    ra.row_gbegin = ctx->row_gbegin.as<uint32_t>();
+    ra.row_gend = ctx->row_gend.as<uint32_t>();
+    ra.row_sign = ctx->row_sign.as<int32_t>();
+    if (R > 0)
+        hipLaunchKernelGGL(k_row_table, dim3(std::min<uint32_t>(grid_for(R, 256), 16384)), dim3(256), 0, s, ra, R);
+    hipLaunchKernelGGL(k_path_rows, dim3(grid_for(NP, 64)), dim3(64), 0, s, grid, NP, ra.row_gbegin, ra.row_gend,
+                       ra.row_sign, ctx->row_wb.as<int32_t>(), ctx->row_path.as<uint32_t>(),
+                       ctx->row_cnt.as<uint64_t>());
+    TileArgs ta;
+    ta.grid = grid;
+    ta.row_path = ctx->row_path.as<uint32_t>();
+    ta.row_gbegin = ra.row_gbegin;
+    ta.row_gend = ra.row_gend;
+    ta.row_wb = ctx->row_wb.as<int32_t>();
+    ta.n_rows = ROWS;
+    ta.gbegin = ctx->grp_begin.as<uint32_t>();
+    ta.gkey = ra.gkey;
+    ta.ghasrun = ctx->grp_hasrun.as<uint32_t>();
+    ta.gsign = ra.gsign;
+    ta.xbits = xbits;
+    ta.vals = svals;
+    ta.recs = ctx->records.as<RunRec>();
+    ta.lines = ctx->lines.as<float4>();
+    ta.row_cnt = ctx->row_cnt.as<uint64_t>();
+    ta.row_off = ctx->row_off.as<uint64_t>();
+    ta.tile_xy = out.tile_xy;
+    ta.tile_alpha = out.tile_alpha;
+    ta.span_xy = out.span_xy;
+    ta.span_w = out.span_w;
+    ta.cnt = cnt;
+    const uint32_t tile_blocks = (uint32_t)std::min<int64_t>(ROWS, 1 << 20);
+    hipLaunchKernelGGL(k_tiles<false>, dim3(tile_blocks), dim3(WAVE), 0, s, ta);
+    HIPCHK(launch_scan<uint64_t>(ctx, ROWS, LoadU64{ta.row_cnt}, StoreU64{ctx->row_off.as<uint64_t>()}, OpAdd(),
+                                 0ull, &tot[4], s));
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(&ht->out, &tot[4], 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(&ht->error, &cnt->error, 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (ht->error & (ERR_BBOX | ERR_REPLAY)) return -3;
+    const int64_t T = (int64_t)(ht->out >> 32);
+    const int64_t S = (int64_t)(uint32_t)ht->out;
+    out.n_tiles = T;
+    out.n_spans = S;
+    if (T > out.tile_capacity || S > out.span_capacity) return -4;
+
+    // ---- (c)(d)(e)
+    if (ctx->timing) {
+        if (!ctx->ev[0]) HIPCHK(hipEventCreate(&ctx->ev[0]));
+        if (!ctx->ev[1]) HIPCHK(hipEventCreate(&ctx->ev[1]));
+        HIPCHK(hipEventRecord(ctx->ev[0], s));
+    }
+    hipLaunchKernelGGL(k_tiles<true>, dim3(tile_blocks), dim3(WAVE), 0, s, ta);
+    if (ctx->timing) HIPCHK(hipEventRecord(ctx->ev[1], s));
+    hipLaunchKernelGGL(k_path_out, dim3(grid_for(NP, 256)), dim3(256), 0, s, grid, NP, ta.row_off,
+                       (const uint64_t*)&tot[4], out.path_tile_start, out.path_tile_count, out.path_span_start,
+                       out.path_span_count, out.tile_xy, out.tile_alpha);
+    HIPCHK(hipGetLastError());
+    if (ctx->timing) {
+        HIPCHK(hipEventSynchronize(ctx->ev[1]));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+        auto& t = ctx->timings["k_tiles"];
+        t.first += ms;
+        t.second += 1;
+    }
+    return 0;
+}
+
+}  // namespace ochre

@@ -1,0 +1,196 @@
+"""Device-resident batch interface (torch is used only to hold device memory
+and to name the stream; all computation is in libochre_hip.so)."""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+
+
+def _torch():
+    import torch
+    return torch
+
+
+class Context:
+    """An ochre_context on one device."""
+
+    def __init__(self, device: int = 0):
+        self.lib = _lib.load()
+        h = ctypes.c_void_p()
+        _lib.check(self.lib.ochre_context_create(device, ctypes.byref(h)), "ochre_context_create")
+        self.h = h
+        self.device = device
+
+    def close(self):
+        if self.h:
+            self.lib.ochre_context_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_timing(self, on: bool):
+        _lib.check(self.lib.ochre_context_set_timing(self.h, 1 if on else 0), "set_timing")
+
+    def timing(self, kernel: str):
+        ms = ctypes.c_double()
+        n = ctypes.c_int64()
+        _lib.check(self.lib.ochre_context_timing(self.h, kernel.encode(), ctypes.byref(ms), ctypes.byref(n)),
+                   "timing")
+        return ms.value, n.value
+
+
+@dataclass
+class DeviceBatch:
+    """A PathBatch uploaded to device memory."""
+    n_paths: int
+    verbs: object
+    verb_off: object
+    points: object
+    point_off: object
+    weights: object
+    weight_off: object
+    xforms: object
+    n_verbs: int
+    n_points: int
+    n_weights: int
+    host: object = None
+
+    def c(self) -> _lib.PathBatchC:
+        def p(t):
+            return t.data_ptr() if t.numel() else None
+        return _lib.PathBatchC(self.n_paths, p(self.verbs), p(self.verb_off), p(self.points), p(self.point_off),
+                               p(self.weights), p(self.weight_off), p(self.xforms), self.n_verbs, self.n_points,
+                               self.n_weights)
+
+
+def upload(batch, device="cuda") -> DeviceBatch:
+    torch = _torch()
+
+    def t(a, dt):
+        a = np.ascontiguousarray(a)
+        if a.size == 0:
+            return torch.zeros(1, dtype=dt, device=device)[:0]
+        return torch.from_numpy(a).to(device)
+
+    return DeviceBatch(
+        n_paths=batch.n_paths,
+        verbs=t(batch.verbs.astype(np.uint8), torch.uint8),
+        verb_off=t(batch.verb_off.astype(np.int64), torch.int64),
+        points=t(batch.points.astype(np.float32).reshape(-1), torch.float32),
+        point_off=t(batch.point_off.astype(np.int64), torch.int64),
+        weights=t(batch.weights.astype(np.float32), torch.float32),
+        weight_off=t(batch.weight_off.astype(np.int64), torch.int64),
+        xforms=t(batch.xforms.astype(np.float32).reshape(-1), torch.float32),
+        n_verbs=int(batch.verbs.size),
+        n_points=int(batch.points.shape[0]),
+        n_weights=int(batch.weights.size),
+        host=batch,
+    )
+
+
+class DeviceOutput:
+    """Caller-owned output arrays (see ochre_batch_output)."""
+
+    def __init__(self, n_paths: int, tile_capacity: int, span_capacity: int, device="cuda"):
+        torch = _torch()
+        self.n_paths = n_paths
+        self.tile_capacity = max(int(tile_capacity), 1)
+        self.span_capacity = max(int(span_capacity), 1)
+        self.tile_xy = torch.empty((self.tile_capacity, 2), dtype=torch.int16, device=device)
+        self.tile_alpha = torch.empty((self.tile_capacity, 64), dtype=torch.uint8, device=device)
+        self.span_xy = torch.empty((self.span_capacity, 2), dtype=torch.int16, device=device)
+        self.span_w = torch.empty((self.span_capacity,), dtype=torch.int16, device=device)
+        n = max(n_paths, 1)
+        self.path_tile_start = torch.empty((n,), dtype=torch.int64, device=device)
+        self.path_tile_count = torch.empty((n,), dtype=torch.int32, device=device)
+        self.path_span_start = torch.empty((n,), dtype=torch.int64, device=device)
+        self.path_span_count = torch.empty((n,), dtype=torch.int32, device=device)
+        self.n_tiles = 0
+        self.n_spans = 0
+
+    def c(self) -> _lib.BatchOutputC:
+        return _lib.BatchOutputC(self.tile_xy.data_ptr(), self.tile_alpha.data_ptr(), self.tile_capacity,
+                                 self.span_xy.data_ptr(), self.span_w.data_ptr(), self.span_capacity,
+                                 self.path_tile_start.data_ptr(), self.path_tile_count.data_ptr(),
+                                 self.path_span_start.data_ptr(), self.path_span_count.data_ptr(), 0, 0)
+
+    def to_host(self):
+        """Per-path host arrays in path order: (tile_xy, tile_alpha, span_xy,
+        span_w, path_tile_off, path_span_off) with contiguous path ranges."""
+        ts = self.path_tile_start[:self.n_paths].cpu().numpy()
+        tc = self.path_tile_count[:self.n_paths].cpu().numpy().astype(np.int64)
+        ss = self.path_span_start[:self.n_paths].cpu().numpy()
+        sc = self.path_span_count[:self.n_paths].cpu().numpy().astype(np.int64)
+        txy = self.tile_xy[:self.n_tiles].cpu().numpy()
+        ta = self.tile_alpha[:self.n_tiles].cpu().numpy()
+        sxy = self.span_xy[:self.n_spans].cpu().numpy()
+        sw = self.span_w[:self.n_spans].cpu().numpy().view(np.uint16)
+        tidx = np.concatenate([np.arange(s, s + c) for s, c in zip(ts, tc)]) if self.n_paths else np.zeros(0, int)
+        sidx = np.concatenate([np.arange(s, s + c) for s, c in zip(ss, sc)]) if self.n_paths else np.zeros(0, int)
+        tidx = tidx.astype(np.int64)
+        sidx = sidx.astype(np.int64)
+        pto = np.zeros(self.n_paths + 1, np.int64)
+        pso = np.zeros(self.n_paths + 1, np.int64)
+        np.cumsum(tc, out=pto[1:])
+        np.cumsum(sc, out=pso[1:])
+        return txy[tidx], ta[tidx], sxy[sidx], sw[sidx], pto, pso
+
+
+def _stream_ptr(stream):
+    torch = _torch()
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    return ctypes.c_void_p(stream.cuda_stream)
+
+
+def rasterize_plain(ctx: Context, db: DeviceBatch, out: DeviceOutput, stream=None):
+    """Runs the five-kernel pipeline.  Returns (n_tiles, n_spans).  Raises
+    OchreError(OCHRE_ERR_CAPACITY) with out.n_tiles/n_spans set when the
+    output arrays are too small."""
+    cb = db.c()
+    co = out.c()
+    st = ctx.lib.ochre_rasterize_plain(ctx.h, ctypes.byref(cb), ctypes.byref(co), _stream_ptr(stream))
+    out.n_tiles, out.n_spans = co.n_tiles, co.n_spans
+    _lib.check(st, "ochre_rasterize_plain")
+    return co.n_tiles, co.n_spans
+
+
+def rasterize_batched(ctx: Context, db: DeviceBatch, out: DeviceOutput, stream=None):
+    """Runs the persistent work-queue pipeline.  Returns (n_tiles, n_spans)."""
+    if not hasattr(ctx.lib, "ochre_rasterize_batched"):
+        raise RuntimeError("libochre_hip.so has no ochre_rasterize_batched")
+    cb = db.c()
+    co = out.c()
+    st = ctx.lib.ochre_rasterize_batched(ctx.h, ctypes.byref(cb), ctypes.byref(co), _stream_ptr(stream))
+    out.n_tiles, out.n_spans = co.n_tiles, co.n_spans
+    _lib.check(st, "ochre_rasterize_batched")
+    return co.n_tiles, co.n_spans
+
+
+def run_to_host(ctx, batch, mode="plain", tile_capacity=None, span_capacity=None):
+    """Convenience: upload, rasterise (growing the outputs once if needed)
+    and return the per-path host arrays."""
+    torch = _torch()
+    db = upload(batch)
+    tc = tile_capacity or max(1024, batch.n_paths * 64)
+    sc = span_capacity or max(1024, batch.n_paths * 8)
+    fn = rasterize_plain if mode == "plain" else rasterize_batched
+    for _ in range(2):
+        out = DeviceOutput(batch.n_paths, tc, sc)
+        try:
+            fn(ctx, db, out)
+            break
+        except _lib.OchreError as e:
+            if e.status != -4:
+                raise
+            tc, sc = out.n_tiles, out.n_spans
+    torch.cuda.synchronize()
+    return out.to_host()

@@ -1,0 +1,132 @@
+"""Parity of the plain five-kernel pipeline with the oracle, through the C ABI."""
+import numpy as np
+import pytest
+
+import oracle_ref
+from ochre_amd import workloads as W
+from parity import compare
+
+pytestmark = pytest.mark.gpu
+
+
+def run_plain(ctx, batch):
+    from ochre_amd.device import run_to_host
+    return run_to_host(ctx, batch, "plain")
+
+
+def check(ctx, batch, max_lsb=1, min_exact=None):
+    print(f"[plain] {batch.name} paths={batch.n_paths}", flush=True)
+    ref = oracle_ref.run_batch(batch)
+    got = run_plain(ctx, batch)
+    st = compare(ref, got, max_lsb=max_lsb)
+    print(f"[plain] {batch.name}: {st}", flush=True)
+    if min_exact is not None:
+        assert st["bit_exact_fraction"] >= min_exact
+    return st
+
+
+def test_readme_path(gpu_ctx):
+    st = check(gpu_ctx, W.readme_path(), max_lsb=0)
+    assert st["tiles"] == 100 and st["spans"] == 23
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_random_paths(gpu_ctx, seed):
+    check(gpu_ctx, W.random_paths(300, seed), max_lsb=0)
+
+
+def test_random_paths_identity_integer(gpu_ctx):
+    # integer and half-integer coordinates exercise DDA ties and zero-length steps
+    check(gpu_ctx, W.random_paths(300, 7, integer_prob=1.0, xform="identity"), max_lsb=0)
+
+
+def test_glyph_run_full(gpu_ctx):
+    check(gpu_ctx, W.glyph_run(), max_lsb=0)
+
+
+def test_star_blobs_small(gpu_ctx):
+    b = W.star_blobs()
+    order = np.argsort(np.diff(b.verb_off))
+    check(gpu_ctx, b.subset(order[:3]), max_lsb=0)
+
+
+def test_adjoint_glyphs(gpu_ctx):
+    check(gpu_ctx, W.adjoint_glyphs(), max_lsb=0)
+
+
+def test_edge_cases(gpu_ctx):
+    M, L, Q, C, K, Z = W.MOVE, W.LINE, W.QUAD, W.CUBIC, W.CONIC, W.CLOSE
+    paths = [
+        ([], []),                                            # empty: one zero tile at (0, 0)
+        ([M], [(5, 5)]),                                     # a lone Move
+        ([M, M, M], [(1, 1), (9, 9), (20, 3)]),              # Moves only: closing lines of zero length
+        ([L, L], [(30, 10), (10, 30)]),                      # no Move: starts at the origin
+        ([M, L, Z, L, Z], [(3, 3), (40, 3), (3, 40)]),       # Close is a no-op
+        ([M, L, L, L], [(8, 8), (72, 8), (72, 72), (8, 72)]),  # integer square
+        ([M, L, L, L], [(8.5, 8.5), (40.5, 8.5), (40.5, 40.5), (8.5, 40.5)]),
+        ([M, L, L, L, M, L, L, L], [(0, 0), (64, 0), (64, 64), (0, 64), (16, 16), (48, 16), (48, 48), (16, 48)]),
+        ([M, L, L, L, M, L, L, L], [(0, 0), (64, 0), (64, 64), (0, 64), (16, 16), (16, 48), (48, 48), (48, 16)]),
+        ([M, L, L], [(-40.25, -3.5), (-3.75, -60.5), (-20, 5)]),  # negative coordinates
+        ([M, Q, Q], [(10, 10), (10, 10), (10, 10), (50, 10), (90, 10)]),  # degenerate quadratics
+        ([M, C, Z], [(10, 10), (10, 10), (10, 10), (10, 10)]),  # a point-cubic
+        ([M, K, K], [(0, 0), (50, 0), (50, 50), (0, 50), (0, 0)]),  # conics
+        ([M, L, L, L], [(31990, 31990), (32100, 31995), (32200, 32300), (31000, 31000)]),  # past i16
+        ([M, L, L], [(100, 100), (100.0000001, 300), (100, 500)]),  # nearly vertical
+        ([M, L, L], [(0.5, 0.5), (4000.5, 1.5), (3.5, 3000.25)]),  # long lines
+    ]
+    weights = {12: [0.7, 2.5]}
+    items = []
+    for i, (v, p) in enumerate(paths):
+        items.append((v, np.asarray(p, np.float32).reshape(-1, 2), weights.get(i, [])))
+    b = W.from_paths(items, W.IDENTITY)
+    b.name = "edge_cases"
+    check(gpu_ctx, b, max_lsb=0)
+
+
+def test_rasterizer_api_event_order(gpu_ctx):
+    """The host Rasterizer replays tiles and spans in the reference's order."""
+    from ochre_amd import EventRecorder, Rasterizer
+    for batch in (W.readme_path(), W.random_paths(20, 11)):
+        for i in range(batch.n_paths):
+            v, p, w, xf = batch.path(i)
+            r = Rasterizer()
+            r.fill_arrays(v, p, w, xf)
+            rec = EventRecorder()
+            r.finish(rec)
+            o = oracle_ref.Rasterizer()
+            o.fill(v, p, w, xf)
+            ref = o.finish()
+            assert len(rec.events) == len(ref), (batch.name, i)
+            for a, b in zip(rec.events, ref):
+                assert a[:3] == b[:3], (batch.name, i, a[:3], b[:3])
+                if a[0] == "tile":
+                    assert np.array_equal(a[3], b[3])
+                else:
+                    assert a[3] == b[3]
+
+
+def test_rasterizer_api_raw_commands_and_multiple_fills(gpu_ctx):
+    """move_to/line_to/command (untransformed) and fill (transformed)
+    segments accumulate into one path, as in the reference."""
+    from ochre_amd import EventRecorder, Rasterizer
+    xf = np.array([1.5, 0.25, -0.5, 1.25, 30.0, 12.0], np.float32)
+    r = Rasterizer()
+    o = oracle_ref.Rasterizer()
+    for rr in (r, o):
+        rr.move_to(10.5, 10.25) if rr is o else rr.move_to(__import__("ochre_amd").Vec2(10.5, 10.25))
+    r.line_to(__import__("ochre_amd").Vec2(60.0, 14.0)); o.line_to(60.0, 14.0)
+    v = np.array([W.QUAD, W.CUBIC], np.uint8)
+    p = np.array([(70, 40), (40, 70), (20, 60), (5, 50), (12, 30)], np.float32)
+    r.fill_arrays(v, p, [], xf); o.fill(v, p, [], xf)
+    from ochre_amd import PathCmd, Vec2
+    r.command(PathCmd.Conic(Vec2(80, 80), Vec2(20, 90), 1.7)); o.command(W.CONIC, [(80, 80), (20, 90)], 1.7)
+    v2 = np.array([W.MOVE, W.LINE, W.LINE], np.uint8)
+    p2 = np.array([(100, 100), (140, 105), (120, 150)], np.float32)
+    r.fill_arrays(v2, p2, [], None); o.fill(v2, p2, [], None)
+    rec = EventRecorder()
+    r.finish(rec)
+    ref = o.finish()
+    assert [e[:3] for e in rec.events] == [e[:3] for e in ref]
+    for a, b in zip(rec.events, ref):
+        if a[0] == "tile":
+            assert np.array_equal(a[3], b[3])
