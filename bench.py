@@ -1,0 +1,189 @@
+"""Throughput of the ochre rasterisation path on one MI355X.
+
+A step is one pass of the hot path (fill -> finish for every path) over one
+batch of synthetic paths already resident in HBM: by default the glyph run
+(10,000 paths, 1-4 closed contours of 8-40 segments, 40% line / 40%
+quadratic / 20% cubic, 12-48 px, 4096x4096 canvas).  Both pipelines are
+timed: `plain` (the five-kernel pipeline) and `batched` (the persistent
+work-queue kernel replayed from a HIP graph); `value` is the batched rate.
+
+Prints one JSON line (the bench contract) to stdout; details go to stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "glyph paths/s + Mtiles/s on 1 MI355X (batched vs plain), % of HBM peak"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(batch, n_tiles, n_spans):
+    """Bytes the path's interface must move: the input path (verbs 1 B,
+    points 8 B, weights 4 B, transform 24 B per path) and the output (64 B
+    alpha + 4 B position per tile, 6 B per span, 24 B of per-path ranges)."""
+    b_in = batch.algorithmic_input_bytes()
+    b_out = 68 * n_tiles + 6 * n_spans + 24 * batch.n_paths
+    return b_in, b_out
+
+
+def cpu_baseline(batch, seconds):
+    """The oracle (a sequential C port of the reference Rasterizer) on one
+    host core, repeated over the same batch until `seconds` have elapsed."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_ref
+    try:
+        os.sched_setaffinity(0, {sorted(os.sched_getaffinity(0))[0]})
+    except Exception:
+        pass
+    oracle_ref.load()
+    reps, t0 = 0, time.perf_counter()
+    times = []
+    while True:
+        a = time.perf_counter()
+        oracle_ref.run_batch(batch)
+        times.append(time.perf_counter() - a)
+        reps += 1
+        if time.perf_counter() - t0 >= seconds and reps >= 2:
+            break
+    times.sort()
+    med = times[len(times) // 2]
+    return {"value": batch.n_paths / med, "unit": "paths/s", "cores": 1, "kind": "port",
+            "sample": f"{batch.name}: all {batch.n_paths} paths, median of {reps} passes (single thread)"}
+
+
+def time_steps(fn, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    s = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(s)
+    for _ in range(steps):
+        fn()
+    e1.record(s)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return e0.elapsed_time(e1) / steps, wall / steps * 1e3
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="glyph_run")
+    ap.add_argument("--mode", default="auto", choices=["auto", "plain", "batched"])
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build's path runs on a single MI355X "
+              f"(--gpus {args.gpus} is not supported)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from ochre_amd import workloads as W
+    from ochre_amd.device import Context, DeviceOutput, rasterize_batched, rasterize_plain, upload
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    batch = W.WORKLOADS[args.workload]()
+    log(f"workload {batch.name}: paths={batch.n_paths} verbs={batch.verbs.size} points={batch.points.shape[0]}")
+    ctx = Context(0)
+    db = upload(batch)
+    torch.cuda.synchronize()
+
+    # size the outputs with one call
+    out = DeviceOutput(batch.n_paths, batch.n_paths * 64, batch.n_paths * 8)
+    try:
+        rasterize_plain(ctx, db, out)
+    except Exception as e:  # capacity: grow once
+        if getattr(e, "status", None) != -4:
+            raise
+        out = DeviceOutput(batch.n_paths, out.n_tiles, out.n_spans)
+        rasterize_plain(ctx, db, out)
+    torch.cuda.synchronize()
+    n_tiles, n_spans = out.n_tiles, out.n_spans
+    b_in, b_out = algorithmic_bytes(batch, n_tiles, n_spans)
+    log(f"tiles={n_tiles} spans={n_spans} algorithmic bytes in={b_in} out={b_out}")
+
+    have_batched = hasattr(ctx.lib, "ochre_rasterize_batched")
+    res = {}
+    modes = ["plain"] + (["batched"] if have_batched and args.mode != "plain" else [])
+    if args.mode == "batched" and not have_batched:
+        raise SystemExit("batched pipeline not built")
+    for mode in modes:
+        fn = (lambda: rasterize_plain(ctx, db, out)) if mode == "plain" else (lambda: rasterize_batched(ctx, db, out))
+        ctx.set_timing(True)
+        ev_ms, wall_ms = time_steps(fn, args.steps, args.warmup)
+        kname = "k_tiles" if mode == "plain" else "k_fused"
+        kms, kn = ctx.timing(kname)
+        ctx.set_timing(False)
+        res[mode] = {"ms": ev_ms, "wall_ms": wall_ms, "kernel": kname,
+                     "kernel_ms": kms / max(kn, 1), "kernel_launches": kn}
+        log(f"{mode}: {ev_ms:.3f} ms/step (wall {wall_ms:.3f}), {batch.n_paths / ev_ms * 1e3:,.0f} paths/s, "
+            f"{kname} {kms / max(kn, 1):.3f} ms x {kn}")
+
+    head = "batched" if "batched" in res else "plain"
+    ms = res[head]["ms"]
+    paths_s = batch.n_paths / ms * 1e3
+    kms = res[head]["kernel_ms"]
+    # roofline of the dominant kernel: the fused kernel moves the whole
+    # interface (input paths + output tiles/spans); the plain pipeline's tile
+    # kernel writes the output and reads nothing of the interface.
+    alg = (b_in + b_out) if head == "batched" else b_out
+    achieved = alg / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
+    line = {
+        "metric": METRIC,
+        "value": paths_s,
+        "unit": "paths/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": batch.name, "paths": batch.n_paths, "canvas": 4096,
+                   "segments": "1-4 contours x 8-40 (40% line, 40% quad, 20% cubic)", "size_px": "12-48",
+                   "pipeline": head},
+        "mtiles_per_s": n_tiles / ms * 1e-3,
+        "tiles": n_tiles,
+        "spans": n_spans,
+        "plain_paths_per_s": batch.n_paths / res["plain"]["ms"] * 1e3,
+        "batched_paths_per_s": (batch.n_paths / res["batched"]["ms"] * 1e3) if "batched" in res else None,
+        "batched_over_plain": (res["plain"]["ms"] / res["batched"]["ms"]) if "batched" in res else None,
+        "hbm_frac_end_to_end": (b_in + b_out) / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": res[head]["kernel"],
+                     "kernel_ms": kms, "algorithmic_bytes": alg},
+    }
+    if not args.no_cpu:
+        cb = cpu_baseline(batch, args.cpu_seconds)
+        cb["cpu"] = platform.processor() or platform.machine()
+        line["cpu_baseline"] = cb
+        line["vs_cpu_1core"] = paths_s / cb["value"]
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -152,6 +152,21 @@ void ochre_context_destroy(ochre_context* ctx);
 int ochre_rasterize_plain(ochre_context* ctx, const ochre_path_batch* in, ochre_batch_output* out,
                           void* stream);
 
+/* The batched pipeline: one persistent work-queue launch (one wave64 per
+ * path; the launch and its queue reset are replayed from a captured HIP
+ * graph), for batches of many small paths.  Paths whose tile grid exceeds
+ * 32 x 32 tiles or which flatten to more than 8192 lines are handed to the
+ * plain pipeline within the same call.  Tiles of a path are contiguous and in
+ * row-major order, but paths are laid out in the order they finish. */
+int ochre_rasterize_batched(ochre_context* ctx, const ochre_path_batch* in, ochre_batch_output* out,
+                            void* stream);
+
+/* Tunables of the batched pipeline (the autotuner sets them): persistent
+ * waves per compute unit (1-32), paths taken per queue pop (1-1024), and
+ * whether to replay the launch from a HIP graph. */
+int ochre_context_set_batched_params(ochre_context* ctx, int waves_per_cu, int paths_per_pop, int use_graph);
+int ochre_context_get_batched_params(ochre_context* ctx, int* waves_per_cu, int* paths_per_pop, int* use_graph);
+
 /* Kernel-time accounting for measurement: when enabled, the pipeline
  * records HIP events around its dominant kernel; ochre_context_timing
  * returns the summed milliseconds and launch count since the last reset. */

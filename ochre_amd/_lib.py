@@ -25,7 +25,8 @@ EXPORTS = [
     "ochre_rasterizer_command", "ochre_rasterizer_fill", "ochre_rasterizer_finish",
     "ochre_rasterizer_free", "ochre_context_create", "ochre_context_destroy",
     "ochre_rasterize_plain", "ochre_context_set_timing", "ochre_context_timing",
-    "ochre_build_info",
+    "ochre_build_info", "ochre_rasterize_batched", "ochre_context_set_batched_params",
+    "ochre_context_get_batched_params",
 ]
 
 
@@ -101,6 +102,9 @@ def _optional(lib):
     P = ctypes.c_void_p
     if hasattr(lib, "ochre_rasterize_batched"):
         lib.ochre_rasterize_batched.argtypes = [P, ctypes.POINTER(PathBatchC), ctypes.POINTER(BatchOutputC), P]
+        lib.ochre_context_set_batched_params.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        lib.ochre_context_get_batched_params.argtypes = [P, ctypes.POINTER(ctypes.c_int),
+                                                         ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
 
 
 def check(status: int, what: str):

@@ -1,0 +1,769 @@
+// Batched pipeline: one persistent launch works through a queue of paths,
+// one wave64 per path, everything for a path on one CU:
+//   flatten   lane per command (64 at a time, wave scans carry the point /
+//             conic / last-Move state), lines to per-wave scratch
+//   count     lane per line, lanes refilled from the line list as they finish
+//             (no lane idles while one long line is walked); marks occupied
+//             tiles and sums tile-row crossings in LDS
+//   allocate  tile and span counts -> one atomic per path for its output
+//             range; spans written directly (integer winding)
+//   emit      lane per line again, each increment (pixel, area, height)
+//             written to scratch at its emission index
+//   commit    64 increments at a time in emission order into LDS tile
+//             accumulators; lanes hitting the same pixel are found by ballot
+//             matching and commit in lane order, so every pixel's f32 sum is
+//             taken in the reference's emission order (rasterizer.rs:221-228)
+//   finalize  row carry (rasterizer.rs:233-250) and quantisation, 64 B stores
+// Paths too large for a wave (grid > 32x32 tiles or > 8192 lines) are
+// deferred to the plain pipeline by the host driver.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "device_common.hpp"
+#include "pipeline.hpp"
+#include "raster_common.hpp"
+#include "scan.hpp"
+
+namespace ochre {
+
+constexpr int F_LCAP = 8192;   // lines per path
+constexpr int F_ICAP = 8192;   // increments per commit batch
+constexpr int F_GRID = 32;     // tile grid side (256 px)
+constexpr int F_TC = 32;       // tiles per accumulator chunk
+constexpr int F_TSTRIDE = 65;  // accumulator stride per tile (bank spread)
+constexpr size_t F_SCRATCH = (size_t)F_LCAP * 16 + (size_t)(F_LCAP + 1) * 4 + 12 + (size_t)F_ICAP * 16;
+constexpr uint32_t F_INVALID = 0xffffffffu;
+
+struct FusedIn {
+    int64_t n_paths;
+    const uint8_t* verbs;
+    const int64_t* verb_off;
+    const float* points;
+    const int64_t* point_off;
+    const float* weights;
+    const int64_t* weight_off;
+    const Xform* xf;
+    int64_t n_points, n_weights;
+};
+
+struct FusedOut {
+    int16_t* tile_xy;
+    uint8_t* tile_alpha;
+    int64_t tile_cap;
+    int16_t* span_xy;
+    uint16_t* span_w;
+    int64_t span_cap;
+    int64_t* pts;
+    int32_t* ptc;
+    int64_t* pss;
+    int32_t* psc;
+};
+
+struct FusedCtl {
+    uint32_t queue;
+    uint32_t deferred;
+    uint32_t error;
+    uint32_t pad0;
+    unsigned long long tiles;
+    unsigned long long spans;
+    uint32_t pad[56];
+};
+
+__global__ void k_fused_reset(FusedCtl* c) {
+    if (threadIdx.x == 0) {
+        c->queue = 0;
+        c->deferred = 0;
+        c->error = 0;
+        c->tiles = 0;
+        c->spans = 0;
+    }
+}
+
+struct OpMax64 {
+    __device__ int64_t operator()(int64_t a, int64_t b) const { return a > b ? a : b; }
+};
+
+__device__ inline V2 fload(const FusedIn& in, int64_t g, const Xform& X) {
+    return apply(X, mk(in.points[2 * g], in.points[2 * g + 1]));
+}
+
+struct LdsState {
+    uint32_t occ[F_GRID];
+    int32_t tsign[F_GRID * F_GRID];
+    uint32_t row_rank[F_GRID + 1];
+    uint32_t chunk_tile[F_TC];
+    float acc_a[F_TC * F_TSTRIDE];
+    float acc_h[F_TC * F_TSTRIDE];
+    float cin[F_TC * 8];
+};
+
+// Rank of tile (tx, ty) (grid-local) among the path's occupied tiles in
+// row-major order.
+__device__ inline uint32_t tile_rank(const LdsState& s, int tx, int ty) {
+    return s.row_rank[ty] + (uint32_t)__popc(s.occ[ty] & ((1u << tx) - 1u));
+}
+
+__device__ void fused_path(const FusedIn& in, const FusedOut& out, FusedCtl* ctl, uint32_t* deferred,
+                           uint8_t* sc, LdsState& s, int64_t p) {
+    const int lane = threadIdx.x;
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    float4* lines = reinterpret_cast<float4*>(sc);
+    uint32_t* incoff = reinterpret_cast<uint32_t*>(sc + (size_t)F_LCAP * 16);
+    float4* incs = reinterpret_cast<float4*>(sc + (size_t)F_LCAP * 16 + (size_t)(F_LCAP + 1) * 4 + 12);
+
+    const int64_t vb = in.verb_off[p], ve = in.verb_off[p + 1];
+    const int64_t pb = in.point_off[p], pe = in.point_off[p + 1];
+    const int64_t wb = in.weight_off[p], we = in.weight_off[p + 1];
+    const Xform X = in.xf[p];
+
+    // ---------------------------------------------------------- flatten
+    uint32_t L = 0;
+    uint32_t cpts = 0, ccon = 0;
+    int64_t cmove = -1;
+    bool overflow = false;
+    bool bad = false, flat_bad = false;
+    BoxAcc box;
+    bool any_line = false;
+    for (int64_t cb = vb; cb < ve; cb += WAVE) {
+        const int64_t i = cb + lane;
+        const bool valid = i < ve;
+        int verb = valid ? in.verbs[i] : (int)CLOSE;
+        int np = verb_points(verb);
+        bool lane_bad = valid && np < 0;
+        if (np < 0) np = 0;
+        const uint32_t np_incl = wave_incl_scan((uint32_t)np, OpAdd());
+        const uint32_t cc = (valid && verb == CONIC) ? 1u : 0u;
+        const uint32_t cc_incl = wave_incl_scan(cc, OpAdd());
+        const int64_t g = pb + cpts + (np_incl - (uint32_t)np);
+        const int64_t mv = (valid && verb == MOVE) ? g : (int64_t)-1;
+        const int64_t mv_incl = wave_incl_scan(mv, OpMax64());
+        int64_t mv_excl = shfl_up_any(mv_incl, 1);
+        if (lane == 0) mv_excl = -1;
+        mv_excl = mv_excl > cmove ? mv_excl : cmove;
+        Cmd c;
+        c.verb = verb;
+        c.path = (int32_t)p;
+        c.last_in_path = (i == ve - 1);
+        c.bad = false;
+        if (valid && g + np > pe) lane_bad = true;
+        if (valid && !lane_bad) {
+            for (int k = 0; k < np; k++) c.pt[k] = fload(in, g + k, X);
+            c.w = 0.0f;
+            if (verb == CONIC) {
+                int64_t wi = wb + ccon + (cc_incl - 1);
+                if (wi < we) c.w = in.weights[wi];
+                else lane_bad = true;
+            }
+            c.start = g > pb ? fload(in, g - 1, X) : mk(0.0f, 0.0f);
+            c.first = mv_excl >= 0 ? fload(in, mv_excl, X) : mk(0.0f, 0.0f);
+        }
+        uint32_t n = 0;
+        if (valid && !lane_bad) {
+            bool ok = command_lines(c, [&](V2 a, V2 b) {
+                n++;
+                box.add(a);
+                box.add(b);
+            });
+            if (!ok) flat_bad = true;
+        }
+        if (n) any_line = true;
+        const uint32_t n_incl = wave_incl_scan(n, OpAdd());
+        const uint32_t chunk_total = __shfl(n_incl, WAVE - 1, WAVE);
+        if (__ballot(lane_bad)) bad = true;
+        if (!overflow && L + chunk_total > (uint32_t)F_LCAP) overflow = true;
+        if (!overflow && valid && !lane_bad && n) {
+            uint32_t o = L + n_incl - n;
+            command_lines(c, [&](V2 a, V2 b) { lines[o++] = make_float4(a.x, a.y, b.x, b.y); });
+        }
+        L += chunk_total;
+        cpts += __shfl(np_incl, WAVE - 1, WAVE);
+        ccon += __shfl(cc_incl, WAVE - 1, WAVE);
+        const int64_t last_mv = shfl_any(mv_incl, WAVE - 1);
+        cmove = last_mv > cmove ? last_mv : cmove;
+    }
+    if (bad || (cpts != (uint32_t)(pe - pb))) {
+        if (lane == 0) atomicOr(&ctl->error, (uint32_t)ERR_INPUT);
+        return;
+    }
+    if (__ballot(flat_bad)) {
+        if (lane == 0) atomicOr(&ctl->error, (uint32_t)ERR_FLATTEN);
+        return;
+    }
+    // path bounding box of all line end points
+    const bool has_lines = __ballot(any_line) != 0;
+    const bool full = __ballot(box.full) != 0;
+    float bminx = box.minx, bminy = box.miny, bmaxx = box.maxx, bmaxy = box.maxy;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        bminx = fminf(bminx, __shfl_xor(bminx, d, WAVE));
+        bminy = fminf(bminy, __shfl_xor(bminy, d, WAVE));
+        bmaxx = fmaxf(bmaxx, __shfl_xor(bmaxx, d, WAVE));
+        bmaxy = fmaxf(bmaxy, __shfl_xor(bmaxy, d, WAVE));
+    }
+    int32_t tx0 = 0, ty0 = 0, GW = 0, GH = 0;
+    if (has_lines && !full && bminx <= bmaxx)
+        grid_from_box(true, false, (int32_t)floorf(bminx), (int32_t)floorf(bminy), (int32_t)floorf(bmaxx),
+                      (int32_t)floorf(bmaxy), &tx0, &ty0, &GW, &GH);
+    if (overflow || full || GW > F_GRID || GH > F_GRID) {
+        if (lane == 0) deferred[atomicAdd(&ctl->deferred, 1u)] = (uint32_t)p;
+        return;
+    }
+
+    // ------------------------------------------------------- count walk
+    for (int k = lane; k < F_GRID * F_GRID; k += WAVE) s.tsign[k] = 0;
+    if (lane < F_GRID) s.occ[lane] = 0;
+    __syncthreads();
+    bool bbox_err = false, walk_err = false;
+    {
+        uint32_t next = 0;
+        bool active = false;
+        Dda d;
+        uint32_t line = 0, cnt = 0, steps = 0;
+        int16_t ty_prev = 0, cur_tx = 0, cur_ty = 0;
+        V2 point = mk(0.0f, 0.0f);
+        while (next < L || __ballot(active)) {
+            const uint64_t need = __ballot(!active);
+            const uint32_t my = next + (uint32_t)__popcll(need & lt);
+            if (!active && my < L) {
+                float4 ln = lines[my];
+                line = my;
+                V2 a = mk(ln.x, ln.y);
+                point = mk(ln.z, ln.w);
+                if (veq(point, a)) {
+                    incoff[line] = 0;
+                } else {
+                    dda_init(d, a, point);
+                    ty_prev = div8(d.y);
+                    cur_tx = div8(d.x);
+                    cur_ty = ty_prev;
+                    int lx = cur_tx - tx0, ly = cur_ty - ty0;
+                    if ((unsigned)lx < (unsigned)GW && (unsigned)ly < (unsigned)GH) atomicOr(&s.occ[ly], 1u << lx);
+                    else bbox_err = true;
+                    cnt = 0;
+                    steps = 0;
+                    active = true;
+                }
+            }
+            next += (uint32_t)__popcll(need);
+            if (active) {
+                cnt++;
+                bool term = dda_step(d);
+                if (term) {
+                    d.x = f2i16(floorf(point.x));
+                    d.y = f2i16(floorf(point.y));
+                }
+                int16_t ty = div8(d.y);
+                if (ty != ty_prev) {
+                    int lx = div8(d.x) - tx0, ly = (ty_prev < ty ? ty_prev : ty) - ty0;
+                    if ((unsigned)lx < (unsigned)GW && (unsigned)ly < (unsigned)GH)
+                        atomicAdd(&s.tsign[ly * F_GRID + lx], (int)(int8_t)(int16_t)(ty - ty_prev));
+                    else bbox_err = true;
+                    ty_prev = ty;
+                }
+                if (!term && ++steps >= WALK_LIMIT) {
+                    walk_err = true;
+                    term = true;
+                }
+                if (term) {
+                    incoff[line] = cnt;
+                    active = false;
+                } else {
+                    int16_t tx = div8(d.x);
+                    if (tx != cur_tx || ty != cur_ty) {
+                        cur_tx = tx;
+                        cur_ty = ty;
+                        int lx = tx - tx0, ly = ty - ty0;
+                        if ((unsigned)lx < (unsigned)GW && (unsigned)ly < (unsigned)GH)
+                            atomicOr(&s.occ[ly], 1u << lx);
+                        else bbox_err = true;
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (__ballot(bbox_err) || __ballot(walk_err)) {
+        if (lane == 0) atomicOr(&ctl->error, __ballot(bbox_err) ? (uint32_t)ERR_BBOX : (uint32_t)ERR_FLATTEN);
+        return;
+    }
+
+    // --------------------------------------------- tiles, spans, output
+    // lane = tile row
+    const bool row_lane = lane < GH;
+    const uint32_t row_tiles = row_lane ? (uint32_t)__popc(s.occ[lane]) : 0u;
+    const uint32_t rt_incl = wave_incl_scan(row_tiles, OpAdd());
+    if (row_lane) s.row_rank[lane] = rt_incl - row_tiles;
+    const uint32_t NT = __shfl(rt_incl, WAVE - 1, WAVE);
+    int32_t row_sum = 0;
+    if (row_lane)
+        for (int x = 0; x < GW; x++) row_sum += s.tsign[lane * F_GRID + x];
+    const int32_t rs_incl = wave_incl_scan(row_sum, OpAdd());
+    const int32_t w_before = rs_incl - row_sum;  // winding of all crossings in earlier rows
+    // spans (rasterizer.rs:252-264): between consecutive tiles of a row
+    uint32_t row_spans = 0;
+    if (row_lane) {
+        int32_t wnd = w_before, w_prev = 0, prev = -1;
+        const uint32_t o = s.occ[lane];
+        for (int x = 0; x < GW; x++) {
+            wnd += s.tsign[lane * F_GRID + x];
+            if ((o >> x) & 1u) {
+                if (prev >= 0 && x > prev + 1 && w_prev != 0) row_spans++;
+                prev = x;
+                w_prev = wnd;
+            }
+        }
+    }
+    const uint32_t rsp_incl = wave_incl_scan(row_spans, OpAdd());
+    const uint32_t NS = __shfl(rsp_incl, WAVE - 1, WAVE);
+    const uint32_t NT_out = NT ? NT : 1u;
+    unsigned long long tb = 0, sb = 0;
+    if (lane == 0) {
+        tb = atomicAdd(&ctl->tiles, (unsigned long long)NT_out);
+        sb = atomicAdd(&ctl->spans, (unsigned long long)NS);
+        out.pts[p] = (int64_t)tb;
+        out.ptc[p] = (int32_t)NT_out;
+        out.pss[p] = (int64_t)sb;
+        out.psc[p] = (int32_t)NS;
+    }
+    tb = __shfl(tb, 0, WAVE);
+    sb = __shfl(sb, 0, WAVE);
+    if (tb + NT_out > (unsigned long long)out.tile_cap || sb + NS > (unsigned long long)out.span_cap) {
+        if (lane == 0) atomicOr(&ctl->error, (uint32_t)ERR_CAPACITY);
+        return;
+    }
+    if (row_lane && row_spans) {
+        uint64_t si = sb + (rsp_incl - row_spans);
+        int32_t wnd = w_before, w_prev = 0, prev = -1;
+        const uint32_t o = s.occ[lane];
+        const int32_t ty = ty0 + lane;
+        for (int x = 0; x < GW; x++) {
+            wnd += s.tsign[lane * F_GRID + x];
+            if ((o >> x) & 1u) {
+                if (prev >= 0 && x > prev + 1 && w_prev != 0) {
+                    out.span_xy[2 * si] = (int16_t)((tx0 + prev + 1) * TILE);
+                    out.span_xy[2 * si + 1] = (int16_t)(ty * TILE);
+                    out.span_w[si] = (uint16_t)((uint16_t)(x - prev - 1) * (uint16_t)TILE);
+                    si++;
+                }
+                prev = x;
+                w_prev = wnd;
+            }
+        }
+    }
+    if (NT == 0) {  // no increments: one all-zero tile at (0, 0) (rasterizer.rs:193-198)
+        if (lane < 4) reinterpret_cast<uint4*>(out.tile_alpha + tb * 64)[lane] = make_uint4(0, 0, 0, 0);
+        if (lane == 0) {
+            out.tile_xy[2 * tb] = 0;
+            out.tile_xy[2 * tb + 1] = 0;
+        }
+        return;
+    }
+
+    // ------------------------------------------- increment offsets
+    {
+        uint32_t run = 0;
+        for (uint32_t b = 0; b < L; b += WAVE) {
+            uint32_t j = b + lane;
+            uint32_t c = j < L ? incoff[j] : 0u;
+            uint32_t ci = wave_incl_scan(c, OpAdd());
+            if (j < L) incoff[j] = run + ci - c;
+            run += __shfl(ci, WAVE - 1, WAVE);
+        }
+        if (lane == 0) incoff[L] = run;
+    }
+    __syncthreads();
+
+    // ------------------------------------- chunks of tiles: emit, commit
+    float carry = 0.0f;
+    int carry_row = -1;
+    for (uint32_t r0 = 0; r0 < NT; r0 += F_TC) {
+        const uint32_t nt = NT - r0 < (uint32_t)F_TC ? NT - r0 : (uint32_t)F_TC;
+        if ((uint32_t)lane < nt) {
+            uint32_t rank = r0 + lane;
+            int ty = 0;
+            while (ty + 1 < GH && s.row_rank[ty + 1] <= rank) ty++;
+            uint32_t k = rank - s.row_rank[ty];
+            uint32_t o = s.occ[ty];
+            for (uint32_t q = 0; q < k; q++) o &= o - 1;
+            int tx = __ffs(o) - 1;
+            s.chunk_tile[lane] = ((uint32_t)ty << 8) | (uint32_t)tx;
+        }
+        for (int k = lane; k < F_TC * F_TSTRIDE; k += WAVE) {
+            s.acc_a[k] = 0.0f;
+            s.acc_h[k] = 0.0f;
+        }
+        __syncthreads();
+        uint32_t l0 = 0;
+        while (l0 < L) {
+            // lines [l0, l1) whose increments fit the batch buffer
+            const uint32_t base = incoff[l0];
+            uint32_t lo = l0 + 1, hi = L;
+            while (lo < hi) {
+                uint32_t mid = (lo + hi + 1) >> 1;
+                if (incoff[mid] - base <= (uint32_t)F_ICAP) lo = mid;
+                else hi = mid - 1;
+            }
+            const uint32_t l1 = lo;
+            const uint32_t n_inc = incoff[l1] - base;
+            if (n_inc > (uint32_t)F_ICAP) {  // one line longer than the buffer: the grid cap rules it out
+                if (lane == 0) atomicOr(&ctl->error, (uint32_t)ERR_BBOX);
+                return;
+            }
+            // emit: lane per line, refilled as lines finish
+            {
+                uint32_t next = l0;
+                bool active = false;
+                Dda d;
+                uint32_t pos = 0;
+                int16_t cur_tx = 0, cur_ty = 0;
+                uint32_t slot_base = F_INVALID;
+                while (next < l1 || __ballot(active)) {
+                    const uint64_t need = __ballot(!active);
+                    const uint32_t my = next + (uint32_t)__popcll(need & lt);
+                    if (!active && my < l1) {
+                        float4 ln = lines[my];
+                        V2 a = mk(ln.x, ln.y), b = mk(ln.z, ln.w);
+                        if (!veq(a, b)) {
+                            dda_init(d, a, b);
+                            pos = incoff[my] - base;
+                            cur_tx = (int16_t)0x7fff;
+                            cur_ty = (int16_t)0x7fff;
+                            active = true;
+                        }
+                    }
+                    next += (uint32_t)__popcll(need);
+                    if (active) {
+                        int16_t tx = div8(d.x), ty = div8(d.y);
+                        if (tx != cur_tx || ty != cur_ty) {
+                            cur_tx = tx;
+                            cur_ty = ty;
+                            uint32_t rank = tile_rank(s, tx - tx0, ty - ty0);
+                            slot_base = (rank - r0 < nt) ? (rank - r0) * F_TSTRIDE : F_INVALID;
+                        }
+                        float area, height;
+                        dda_increment(d, area, height);
+                        uint32_t key = slot_base == F_INVALID ? F_INVALID
+                                                              : slot_base + (uint32_t)(((d.y & 7) << 3) | (d.x & 7));
+                        incs[pos++] = make_float4(__uint_as_float(key), area, height, 0.0f);
+                        if (dda_step(d)) active = false;
+                    }
+                }
+            }
+            __syncthreads();
+            // commit in emission order
+            for (uint32_t cb = 0; cb < n_inc; cb += WAVE) {
+                const uint32_t j = cb + lane;
+                float4 v = j < n_inc ? incs[j] : make_float4(__uint_as_float(F_INVALID), 0.0f, 0.0f, 0.0f);
+                const uint32_t key = __float_as_uint(v.x);
+                const bool valid = key != F_INVALID;
+                uint64_t match = __ballot(valid);
+#pragma unroll
+                for (int b = 0; b < 12; b++) {
+                    const bool bit = (key >> b) & 1u;
+                    const uint64_t m = __ballot(bit);
+                    match &= bit ? m : ~m;
+                }
+                const int rank = __popcll(match & lt);
+                for (int r = 0;; r++) {
+                    if (valid && rank == r) {
+                        s.acc_a[key] += v.y;
+                        s.acc_h[key] += v.z;
+                    }
+                    if (!__ballot(valid && rank > r)) break;
+                }
+            }
+            __syncthreads();
+            l0 = l1;
+        }
+        // row carry, one lane per pixel row, sequential over the chunk's tiles
+        if (lane < 8) {
+            for (uint32_t t = 0; t < nt; t++) {
+                int ty = (int)(s.chunk_tile[t] >> 8);
+                if (ty != carry_row) {
+                    carry = 0.0f;
+                    carry_row = ty;
+                }
+                s.cin[t * 8 + lane] = carry;
+#pragma unroll
+                for (int x = 0; x < 8; x++) carry += s.acc_h[t * F_TSTRIDE + lane * 8 + x];
+            }
+        }
+        __syncthreads();
+        // quantise and store (rasterizer.rs:231-241)
+        if ((uint32_t)lane < nt) {
+            const int t = lane;
+            const int ty = (int)(s.chunk_tile[t] >> 8), tx = (int)(s.chunk_tile[t] & 0xffu);
+            uint32_t words[16];
+#pragma unroll
+            for (int y = 0; y < 8; y++) {
+                float accum = s.cin[t * 8 + y];
+                uint32_t w0 = 0, w1 = 0;
+#pragma unroll
+                for (int x = 0; x < 8; x++) {
+                    const int k = t * F_TSTRIDE + y * 8 + x;
+                    float v = fminf(fabsf(accum + s.acc_a[k]) * 256.0f, 255.0f);
+                    uint32_t b = (uint32_t)v;
+                    if (x < 4) w0 |= b << (8 * x);
+                    else w1 |= b << (8 * (x - 4));
+                    accum += s.acc_h[k];
+                }
+                words[2 * y] = w0;
+                words[2 * y + 1] = w1;
+            }
+            const uint64_t ti = tb + r0 + (uint32_t)t;
+            uint4* dst = reinterpret_cast<uint4*>(out.tile_alpha + ti * 64);
+            dst[0] = make_uint4(words[0], words[1], words[2], words[3]);
+            dst[1] = make_uint4(words[4], words[5], words[6], words[7]);
+            dst[2] = make_uint4(words[8], words[9], words[10], words[11]);
+            dst[3] = make_uint4(words[12], words[13], words[14], words[15]);
+            out.tile_xy[2 * ti] = (int16_t)((tx0 + tx) * TILE);
+            out.tile_xy[2 * ti + 1] = (int16_t)((ty0 + ty) * TILE);
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(WAVE) void k_fused(FusedIn in, FusedOut out, FusedCtl* ctl, uint32_t* deferred,
+                                                uint8_t* scratch, uint32_t gran) {
+    __shared__ LdsState s;
+    uint8_t* sc = scratch + (size_t)blockIdx.x * F_SCRATCH;
+    for (;;) {
+        uint32_t base = 0;
+        if (threadIdx.x == 0) base = atomicAdd(&ctl->queue, gran);
+        base = __shfl(base, 0, WAVE);
+        if (base >= (uint64_t)in.n_paths) break;
+        const int64_t end = std::min<int64_t>((int64_t)base + gran, in.n_paths);
+        for (int64_t p = base; p < end; p++) fused_path(in, out, ctl, deferred, sc, s, p);
+    }
+}
+
+// ---------------------------------------------- deferred paths -> plain
+
+// Gathers the deferred paths into a compact segment input.
+__global__ void k_gather_paths(FusedIn in, const uint32_t* ids, uint32_t n, const int64_t* dst_voff,
+                               const int64_t* dst_poff, const int64_t* dst_woff, uint8_t* verbs, float* points,
+                               float* weights, Xform* xf) {
+    for (uint32_t k = blockIdx.x; k < n; k += gridDim.x) {
+        const int64_t p = ids[k];
+        const int64_t vb = in.verb_off[p], nv = in.verb_off[p + 1] - vb;
+        const int64_t pb = in.point_off[p], np = in.point_off[p + 1] - pb;
+        const int64_t wb = in.weight_off[p], nw = in.weight_off[p + 1] - wb;
+        for (int64_t i = threadIdx.x; i < nv; i += blockDim.x) verbs[dst_voff[k] + i] = in.verbs[vb + i];
+        for (int64_t i = threadIdx.x; i < 2 * np; i += blockDim.x) points[2 * dst_poff[k] + i] = in.points[2 * pb + i];
+        for (int64_t i = threadIdx.x; i < nw; i += blockDim.x) weights[dst_woff[k] + i] = in.weights[wb + i];
+        if (threadIdx.x == 0) xf[k] = in.xf[p];
+    }
+}
+
+__global__ void k_scatter_ranges(const uint32_t* ids, uint32_t n, const int64_t* ts, const int32_t* tc,
+                                 const int64_t* ss, const int32_t* scn, int64_t tbase, int64_t sbase, int64_t* pts,
+                                 int32_t* ptc, int64_t* pss, int32_t* psc) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t p = ids[k];
+    pts[p] = ts[k] + tbase;
+    ptc[p] = tc[k];
+    pss[p] = ss[k] + sbase;
+    psc[p] = scn[k];
+}
+
+struct BatchedState {
+    DBuf scratch, ctl, deferred;
+    DBuf g_verbs, g_points, g_weights, g_voff, g_poff, g_woff, g_xf, g_ts, g_tc, g_ss, g_sc;
+    int n_waves = 0;
+    uint32_t gran = 1;
+    // graph of (reset, fused) for the last argument set
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<uint8_t> graph_key;
+    ~BatchedState() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+};
+
+int run_batched(Context* ctx, const SegInput& in, OutputDesc& out, hipStream_t s, int waves_per_cu, int gran,
+                bool use_graph) {
+    if (!ctx->batched) ctx->batched = new BatchedState();
+    BatchedState* B = reinterpret_cast<BatchedState*>(ctx->batched);
+    const int64_t NP = in.n_paths;
+    out.n_tiles = out.n_spans = 0;
+    if (NP <= 0) return 0;
+    int dev = 0;
+    HIPCHK_RET(hipGetDevice(&dev));
+    int cus = 0;
+    HIPCHK_RET(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const int n_waves = std::max(1, cus * std::max(1, waves_per_cu));
+    HIPCHK_RET(B->scratch.ensure((size_t)n_waves * F_SCRATCH));
+    HIPCHK_RET(B->ctl.ensure(sizeof(FusedCtl)));
+    HIPCHK_RET(B->deferred.ensure((size_t)NP * 4));
+    if (!ctx->host_pinned) HIPCHK_RET(hipHostMalloc(&ctx->host_pinned, 4096));
+    FusedIn fi;
+    fi.n_paths = NP;
+    fi.verbs = in.verbs;
+    fi.verb_off = in.seg_verb_off;
+    fi.points = in.points;
+    fi.point_off = in.seg_point_off;
+    fi.weights = in.weights;
+    fi.weight_off = in.seg_weight_off;
+    fi.xf = in.seg_xf;
+    fi.n_points = in.n_points;
+    fi.n_weights = in.n_weights;
+    FusedOut fo;
+    fo.tile_xy = out.tile_xy;
+    fo.tile_alpha = out.tile_alpha;
+    fo.tile_cap = out.tile_capacity;
+    fo.span_xy = out.span_xy;
+    fo.span_w = out.span_w;
+    fo.span_cap = out.span_capacity;
+    fo.pts = out.path_tile_start;
+    fo.ptc = out.path_tile_count;
+    fo.pss = out.path_span_start;
+    fo.psc = out.path_span_count;
+    FusedCtl* ctl = B->ctl.as<FusedCtl>();
+    uint32_t* deferred = B->deferred.as<uint32_t>();
+    uint8_t* scratch = B->scratch.as<uint8_t>();
+    const uint32_t g = (uint32_t)std::max(1, gran);
+
+    auto enqueue = [&](hipStream_t st) {
+        hipLaunchKernelGGL(k_fused_reset, dim3(1), dim3(64), 0, st, ctl);
+        hipLaunchKernelGGL(k_fused, dim3(n_waves), dim3(WAVE), 0, st, fi, fo, ctl, deferred, scratch, g);
+    };
+    const bool timing = ctx->timing;
+    if (timing) {
+        if (!ctx->ev[0]) HIPCHK_RET(hipEventCreate(&ctx->ev[0]));
+        if (!ctx->ev[1]) HIPCHK_RET(hipEventCreate(&ctx->ev[1]));
+    }
+    if (use_graph) {
+        std::vector<uint8_t> key(sizeof(fi) + sizeof(fo) + 16);
+        memcpy(key.data(), &fi, sizeof(fi));
+        memcpy(key.data() + sizeof(fi), &fo, sizeof(fo));
+        memcpy(key.data() + sizeof(fi) + sizeof(fo), &n_waves, 4);
+        memcpy(key.data() + sizeof(fi) + sizeof(fo) + 4, &g, 4);
+        memcpy(key.data() + sizeof(fi) + sizeof(fo) + 8, &scratch, 8);
+        if (!B->exec || key != B->graph_key) {
+            if (B->exec) (void)hipGraphExecDestroy(B->exec);
+            if (B->graph) (void)hipGraphDestroy(B->graph);
+            B->exec = nullptr;
+            B->graph = nullptr;
+            hipStream_t cs;
+            HIPCHK_RET(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+            HIPCHK_RET(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+            enqueue(cs);
+            hipError_t e = hipStreamEndCapture(cs, &B->graph);
+            (void)hipStreamDestroy(cs);
+            HIPCHK_RET(e);
+            HIPCHK_RET(hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0));
+            B->graph_key = key;
+        }
+        if (timing) HIPCHK_RET(hipEventRecord(ctx->ev[0], s));
+        HIPCHK_RET(hipGraphLaunch(B->exec, s));
+        if (timing) HIPCHK_RET(hipEventRecord(ctx->ev[1], s));
+    } else {
+        hipLaunchKernelGGL(k_fused_reset, dim3(1), dim3(64), 0, s, ctl);
+        if (timing) HIPCHK_RET(hipEventRecord(ctx->ev[0], s));
+        hipLaunchKernelGGL(k_fused, dim3(n_waves), dim3(WAVE), 0, s, fi, fo, ctl, deferred, scratch, g);
+        if (timing) HIPCHK_RET(hipEventRecord(ctx->ev[1], s));
+    }
+    HIPCHK_RET(hipGetLastError());
+    FusedCtl* hc = reinterpret_cast<FusedCtl*>(reinterpret_cast<uint8_t*>(ctx->host_pinned) + 1024);
+    HIPCHK_RET(hipMemcpyAsync(hc, ctl, sizeof(FusedCtl), hipMemcpyDeviceToHost, s));
+    HIPCHK_RET(hipStreamSynchronize(s));
+    if (timing) {
+        float ms = 0;
+        HIPCHK_RET(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+        auto& t = ctx->timings["k_fused"];
+        t.first += ms;
+        t.second += 1;
+    }
+    if (hc->error & ERR_INPUT) return -1;
+    if (hc->error & ERR_FLATTEN) return -5;
+    if (hc->error & ERR_BBOX) return -3;
+    const int64_t Tf = (int64_t)hc->tiles, Sf = (int64_t)hc->spans;
+    const uint32_t nd = hc->deferred;
+    int64_t Tp = 0, Sp = 0;
+    int st = 0;
+    if (nd > 0) {
+        // compact the deferred paths and run the plain pipeline on them
+        std::vector<uint32_t> ids(nd);
+        HIPCHK_RET(hipMemcpy(ids.data(), deferred, nd * 4, hipMemcpyDeviceToHost));
+        std::sort(ids.begin(), ids.end());
+        std::vector<int64_t> voff(NP + 1), poff(NP + 1), woff(NP + 1);
+        HIPCHK_RET(hipMemcpy(voff.data(), in.seg_verb_off, (NP + 1) * 8, hipMemcpyDeviceToHost));
+        HIPCHK_RET(hipMemcpy(poff.data(), in.seg_point_off, (NP + 1) * 8, hipMemcpyDeviceToHost));
+        HIPCHK_RET(hipMemcpy(woff.data(), in.seg_weight_off, (NP + 1) * 8, hipMemcpyDeviceToHost));
+        std::vector<int64_t> dv(nd + 1, 0), dp(nd + 1, 0), dw(nd + 1, 0);
+        for (uint32_t k = 0; k < nd; k++) {
+            uint32_t p = ids[k];
+            dv[k + 1] = dv[k] + voff[p + 1] - voff[p];
+            dp[k + 1] = dp[k] + poff[p + 1] - poff[p];
+            dw[k + 1] = dw[k] + woff[p + 1] - woff[p];
+        }
+        HIPCHK_RET(B->g_verbs.ensure(std::max<int64_t>(dv[nd], 1)));
+        HIPCHK_RET(B->g_points.ensure(std::max<int64_t>(dp[nd], 1) * 8));
+        HIPCHK_RET(B->g_weights.ensure(std::max<int64_t>(dw[nd], 1) * 4));
+        HIPCHK_RET(B->g_voff.ensure((nd + 1) * 8));
+        HIPCHK_RET(B->g_poff.ensure((nd + 1) * 8));
+        HIPCHK_RET(B->g_woff.ensure((nd + 1) * 8));
+        HIPCHK_RET(B->g_xf.ensure(nd * sizeof(Xform)));
+        HIPCHK_RET(B->g_ts.ensure(nd * 8));
+        HIPCHK_RET(B->g_tc.ensure(nd * 4));
+        HIPCHK_RET(B->g_ss.ensure(nd * 8));
+        HIPCHK_RET(B->g_sc.ensure(nd * 4));
+        HIPCHK_RET(hipMemcpyAsync(deferred, ids.data(), nd * 4, hipMemcpyHostToDevice, s));
+        HIPCHK_RET(hipMemcpyAsync(B->g_voff.p, dv.data(), (nd + 1) * 8, hipMemcpyHostToDevice, s));
+        HIPCHK_RET(hipMemcpyAsync(B->g_poff.p, dp.data(), (nd + 1) * 8, hipMemcpyHostToDevice, s));
+        HIPCHK_RET(hipMemcpyAsync(B->g_woff.p, dw.data(), (nd + 1) * 8, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_gather_paths, dim3(std::min<uint32_t>(nd, 4096)), dim3(256), 0, s, fi, deferred, nd,
+                           B->g_voff.as<int64_t>(), B->g_poff.as<int64_t>(), B->g_woff.as<int64_t>(),
+                           B->g_verbs.as<uint8_t>(), B->g_points.as<float>(), B->g_weights.as<float>(),
+                           B->g_xf.as<Xform>());
+        HIPCHK_RET(hipGetLastError());
+        SegInput si;
+        si.n_segs = nd;
+        si.n_paths = nd;
+        si.verbs = B->g_verbs.as<uint8_t>();
+        si.points = B->g_points.as<float>();
+        si.weights = B->g_weights.as<float>();
+        si.seg_verb_off = B->g_voff.as<int64_t>();
+        si.seg_point_off = B->g_poff.as<int64_t>();
+        si.seg_weight_off = B->g_woff.as<int64_t>();
+        si.seg_xf = B->g_xf.as<Xform>();
+        si.n_verbs = dv[nd];
+        si.n_points = dp[nd];
+        si.n_weights = dw[nd];
+        OutputDesc od;
+        const bool room = Tf <= out.tile_capacity && Sf <= out.span_capacity;
+        od.tile_xy = out.tile_xy + (room ? 2 * Tf : 0);
+        od.tile_alpha = out.tile_alpha + (room ? 64 * Tf : 0);
+        od.tile_capacity = room ? out.tile_capacity - Tf : 0;
+        od.span_xy = out.span_xy + (room ? 2 * Sf : 0);
+        od.span_w = out.span_w + (room ? Sf : 0);
+        od.span_capacity = room ? out.span_capacity - Sf : 0;
+        od.path_tile_start = B->g_ts.as<int64_t>();
+        od.path_tile_count = B->g_tc.as<int32_t>();
+        od.path_span_start = B->g_ss.as<int64_t>();
+        od.path_span_count = B->g_sc.as<int32_t>();
+        st = run_plain(ctx, si, od, s);
+        Tp = od.n_tiles;
+        Sp = od.n_spans;
+        if (st == 0) {
+            hipLaunchKernelGGL(k_scatter_ranges, dim3((nd + 255) / 256), dim3(256), 0, s, deferred, nd,
+                               od.path_tile_start, od.path_tile_count, od.path_span_start, od.path_span_count, Tf,
+                               Sf, out.path_tile_start, out.path_tile_count, out.path_span_start,
+                               out.path_span_count);
+            HIPCHK_RET(hipGetLastError());
+        } else if (st != -4) {
+            return st;
+        }
+    }
+    out.n_tiles = Tf + Tp;
+    out.n_spans = Sf + Sp;
+    if ((hc->error & ERR_CAPACITY) || out.n_tiles > out.tile_capacity || out.n_spans > out.span_capacity) return -4;
+    return st;
+}
+
+}  // namespace ochre

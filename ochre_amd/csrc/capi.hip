@@ -132,6 +132,76 @@ int ochre_rasterize_plain(ochre_context* ctx, const ochre_path_batch* in, ochre_
     return st;
 }
 
+static int to_seg_input(const ochre_path_batch* in, SegInput* si) {
+    if (in->n_paths < 0 || in->n_verbs < 0 || in->n_points < 0 || in->n_weights < 0) return OCHRE_ERR_INVALID_ARG;
+    if (in->n_paths > 0 && (!in->verb_off || !in->point_off || !in->weight_off || !in->transforms))
+        return OCHRE_ERR_INVALID_ARG;
+    if ((in->n_verbs > 0 && !in->verbs) || (in->n_points > 0 && !in->points) || (in->n_weights > 0 && !in->weights))
+        return OCHRE_ERR_INVALID_ARG;
+    si->n_segs = in->n_paths;
+    si->n_paths = in->n_paths;
+    si->verbs = in->verbs;
+    si->points = in->points;
+    si->weights = in->weights;
+    si->seg_verb_off = in->verb_off;
+    si->seg_point_off = in->point_off;
+    si->seg_weight_off = in->weight_off;
+    si->seg_xf = reinterpret_cast<const Xform*>(in->transforms);
+    si->n_verbs = in->n_verbs;
+    si->n_points = in->n_points;
+    si->n_weights = in->n_weights;
+    return OCHRE_OK;
+}
+
+static OutputDesc to_output(const ochre_batch_output* out) {
+    OutputDesc od;
+    od.tile_xy = out->tile_xy;
+    od.tile_alpha = out->tile_alpha;
+    od.tile_capacity = out->tile_capacity;
+    od.span_xy = out->span_xy;
+    od.span_w = out->span_width;
+    od.span_capacity = out->span_capacity;
+    od.path_tile_start = out->path_tile_start;
+    od.path_tile_count = out->path_tile_count;
+    od.path_span_start = out->path_span_start;
+    od.path_span_count = out->path_span_count;
+    return od;
+}
+
+int ochre_rasterize_batched(ochre_context* ctx, const ochre_path_batch* in, ochre_batch_output* out, void* stream) {
+    if (!ctx || !in || !out) return OCHRE_ERR_INVALID_ARG;
+    SegInput si;
+    int st = to_seg_input(in, &si);
+    if (st) return st;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (hipSetDevice(ctx->c.device) != hipSuccess) return OCHRE_ERR_DEVICE;
+    OutputDesc od = to_output(out);
+    st = run_batched(&ctx->c, si, od, reinterpret_cast<hipStream_t>(stream), ctx->c.batched_waves_per_cu,
+                     ctx->c.batched_gran, ctx->c.batched_graph);
+    out->n_tiles = od.n_tiles;
+    out->n_spans = od.n_spans;
+    return st;
+}
+
+int ochre_context_set_batched_params(ochre_context* ctx, int waves_per_cu, int paths_per_pop, int use_graph) {
+    if (!ctx || waves_per_cu < 1 || waves_per_cu > 32 || paths_per_pop < 1 || paths_per_pop > 1024)
+        return OCHRE_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->c.batched_waves_per_cu = waves_per_cu;
+    ctx->c.batched_gran = paths_per_pop;
+    ctx->c.batched_graph = use_graph != 0;
+    return OCHRE_OK;
+}
+
+int ochre_context_get_batched_params(ochre_context* ctx, int* waves_per_cu, int* paths_per_pop, int* use_graph) {
+    if (!ctx) return OCHRE_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (waves_per_cu) *waves_per_cu = ctx->c.batched_waves_per_cu;
+    if (paths_per_pop) *paths_per_pop = ctx->c.batched_gran;
+    if (use_graph) *use_graph = ctx->c.batched_graph ? 1 : 0;
+    return OCHRE_OK;
+}
+
 }  // extern "C"
 
 // --------------------------------------------------------------- Rasterizer

@@ -89,9 +89,8 @@ struct DBuf {
 struct Context {
     int device = 0;
     hipStream_t own_stream = nullptr;
-    // scan look-back state
-    DBuf scan_flags, scan_agg, scan_incl, scan_ctl;
-    uint32_t scan_cap_blocks = 0;
+    // tile sums of the device-wide scans
+    DBuf scan_partials;
     // pipeline scratch
     DBuf cmd_scan, cmd_seg, cmd_lines, cmd_line_off;
     DBuf lines, line_path, line_cnt, line_off;
@@ -108,22 +107,25 @@ struct Context {
     hipEvent_t ev[2] = {nullptr, nullptr};
     // batched-path state lives in batched.hip
     void* batched = nullptr;
+    int batched_waves_per_cu = 6;
+    int batched_gran = 1;
+    bool batched_graph = true;
     ~Context();
 };
 
 int run_plain(Context* ctx, const SegInput& in, OutputDesc& out, hipStream_t stream);
+// Persistent work-queue pipeline (one wave per path); paths that do not fit a
+// wave are deferred to run_plain.  waves_per_cu sets the persistent grid,
+// gran the paths taken per queue pop; use_graph replays (reset, kernel) from
+// a captured HIP graph.
+int run_batched(Context* ctx, const SegInput& in, OutputDesc& out, hipStream_t stream, int waves_per_cu, int gran,
+                bool use_graph);
 
-// Scan helper bound to a context's look-back state.
-template <typename T>
-ScanState<T> scan_state(Context* ctx) {
-    ScanState<T> s;
-    s.flags = ctx->scan_flags.as<uint32_t>();
-    s.aggregate = ctx->scan_agg.as<T>();
-    s.inclusive = ctx->scan_incl.as<T>();
-    s.ctl = ctx->scan_ctl.as<uint32_t>();
-    return s;
-}
-hipError_t ensure_scan_capacity(Context* ctx, int64_t n);
+#define HIPCHK_RET(x)                                      \
+    do {                                                   \
+        hipError_t e_ = (x);                               \
+        if (e_ != hipSuccess) return ::ochre::map_hip_error(e_); \
+    } while (0)
 
 int map_hip_error(hipError_t e);
 

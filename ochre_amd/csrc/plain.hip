@@ -19,6 +19,7 @@
 #include "device_common.hpp"
 #include "pipeline.hpp"
 #include "radix_sort.hpp"
+#include "raster_common.hpp"
 #include "scan.hpp"
 
 namespace ochre {
@@ -61,37 +62,27 @@ int map_hip_error(hipError_t e) {
 }
 
 hipError_t ensure_scan_capacity(Context* ctx, int64_t n) {
-    uint32_t blocks = (uint32_t)std::max<int64_t>(1, (n + SCAN_TILE - 1) / SCAN_TILE);
-    if (blocks <= ctx->scan_cap_blocks && ctx->scan_ctl.p) return hipSuccess;
-    uint32_t cap = std::max<uint32_t>(blocks, 1024);
-    hipError_t e;
-    // flags keep their epoch tags; fresh memory must start at zero.
-    DBuf nf;
-    if ((e = nf.ensure(cap * sizeof(uint32_t))) != hipSuccess) return e;
-    if ((e = hipMemset(nf.p, 0, nf.bytes)) != hipSuccess) return e;
-    std::swap(ctx->scan_flags.p, nf.p);
-    std::swap(ctx->scan_flags.bytes, nf.bytes);
-    if ((e = ctx->scan_agg.ensure(cap * 16)) != hipSuccess) return e;
-    if ((e = ctx->scan_incl.ensure(cap * 16)) != hipSuccess) return e;
-    if (!ctx->scan_ctl.p) {
-        if ((e = ctx->scan_ctl.ensure(256)) != hipSuccess) return e;
-        if ((e = hipMemset(ctx->scan_ctl.p, 0, 256)) != hipSuccess) return e;
-    } else {
-        // epochs restart with the new flag array
-        if ((e = hipMemset(ctx->scan_ctl.p, 0, 256)) != hipSuccess) return e;
-    }
-    if ((e = hipDeviceSynchronize()) != hipSuccess) return e;
-    ctx->scan_cap_blocks = cap;
-    return hipSuccess;
+    int64_t blocks = std::max<int64_t>(1, (n + SCAN_TILE - 1) / SCAN_TILE);
+    return ctx->scan_partials.ensure((size_t)blocks * 16);
 }
 
 template <typename T, typename Op, typename Load, typename Store>
 static hipError_t launch_scan(Context* ctx, int64_t n, Load load, Store store, Op op, T identity, T* total,
                               hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    uint32_t blocks = (uint32_t)((n + SCAN_TILE - 1) / SCAN_TILE);
-    hipLaunchKernelGGL((k_scan<T, Op, Load, Store>), dim3(blocks), dim3(SCAN_NT), 0, s, load, store, n,
-                       scan_state<T>(ctx), op, identity, total);
+    if (n <= SCAN_TILE) {
+        hipLaunchKernelGGL((k_scan_single<T, Op, Load, Store>), dim3(1), dim3(SCAN_NT), 0, s, load, store, n, op,
+                           identity, total);
+        return hipGetLastError();
+    }
+    const int64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+    if ((size_t)nb * sizeof(T) > ctx->scan_partials.bytes) return hipErrorInvalidValue;
+    T* partials = ctx->scan_partials.as<T>();
+    hipLaunchKernelGGL((k_scan_reduce<T, Op, Load>), dim3((uint32_t)nb), dim3(SCAN_NT), 0, s, load, n, partials, op,
+                       identity);
+    hipLaunchKernelGGL((k_scan_partials<T, Op>), dim3(1), dim3(SCAN_NT), 0, s, partials, nb, op, identity, total);
+    hipLaunchKernelGGL((k_scan_down<T, Op, Load, Store>), dim3((uint32_t)nb), dim3(SCAN_NT), 0, s, load, store, n,
+                       (const T*)partials, op, identity);
     return hipGetLastError();
 }
 
@@ -157,18 +148,6 @@ __device__ inline V2 load_point(const SegInput& in, int64_t g, int64_t s) {
     return apply(in.seg_xf[s], p);
 }
 
-// Resolved command: its transformed points and the rasterizer state
-// (`last`, `first`) it starts from (rasterizer.rs:145-165).
-struct Cmd {
-    int verb;
-    int32_t path;
-    bool last_in_path;
-    bool bad;
-    V2 pt[3];
-    float w;
-    V2 start, first;
-};
-
 __device__ inline Cmd resolve_cmd(const SegInput& in, const CmdAcc* scan, int64_t i) {
     Cmd c;
     int64_t s = seg_search(in.seg_verb_off, in.n_segs, i);
@@ -213,93 +192,6 @@ __device__ inline Cmd resolve_cmd(const SegInput& in, const CmdAcc* scan, int64_
     }
     return c;
 }
-
-// Enumerates the lines a command feeds to line_to (rasterizer.rs:61-69,
-// 145-157 and the finish-time close, :181-183), in order.
-template <class Emit>
-__device__ inline bool command_lines(const Cmd& c, Emit&& emit) {
-    bool ok = true;
-    V2 last = c.start;
-    V2 first = c.first;
-    switch (c.verb) {
-    case MOVE:
-        if (!veq(last, first)) emit(last, first);
-        last = c.pt[0];
-        first = c.pt[0];
-        break;
-    case LINE:
-        emit(last, c.pt[0]);
-        last = c.pt[0];
-        break;
-    case QUAD: {
-        float dt = quad_dt(last, c.pt[0], c.pt[1]);
-        float t = 0.0f;
-        uint32_t n = 0;
-        V2 prev = last;
-        while (t < 1.0f) {
-            t = fminf(t + dt, 1.0f);
-            V2 q = quad_at(t, last, c.pt[0], c.pt[1]);
-            emit(prev, q);
-            prev = q;
-            if (++n >= FLATTEN_LIMIT) {
-                ok = false;
-                break;
-            }
-        }
-        last = prev;
-        break;
-    }
-    case CUBIC: {
-        float dt = cubic_dt(last, c.pt[0], c.pt[1], c.pt[2]);
-        float t = 0.0f;
-        uint32_t n = 0;
-        V2 prev = last;
-        while (t < 1.0f) {
-            t = fminf(t + dt, 1.0f);
-            V2 q = cubic_at(t, last, c.pt[0], c.pt[1], c.pt[2]);
-            emit(prev, q);
-            prev = q;
-            if (++n >= FLATTEN_LIMIT) {
-                ok = false;
-                break;
-            }
-        }
-        last = prev;
-        break;
-    }
-    case CONIC: {
-        V2 prev = last;
-        ok = conic_walk(last, c.pt[0], c.pt[1], c.w, [&](V2 q) {
-            emit(prev, q);
-            prev = q;
-        });
-        last = prev;
-        break;
-    }
-    default:  // CLOSE: a no-op in the rasterizer (rasterizer.rs:154)
-        break;
-    }
-    if (c.last_in_path && !veq(last, first)) emit(last, first);
-    return ok;
-}
-
-constexpr float HUGE_COORD = 30000.0f;
-enum BoxFlags : uint32_t { BOX_FULL = 1u, BOX_LINES = 2u };
-
-struct BoxAcc {
-    float minx = INFINITY, miny = INFINITY, maxx = -INFINITY, maxy = -INFINITY;
-    bool full = false;
-    __device__ void add(V2 p) {
-        if (!(fabsf(p.x) <= HUGE_COORD) || !(fabsf(p.y) <= HUGE_COORD)) {
-            full = true;
-            return;
-        }
-        minx = fminf(minx, p.x);
-        miny = fminf(miny, p.y);
-        maxx = fmaxf(maxx, p.x);
-        maxy = fmaxf(maxy, p.y);
-    }
-};
 
 struct FlattenArgs {
     SegInput in;
@@ -353,34 +245,14 @@ __global__ __launch_bounds__(256) void k_flatten(FlattenArgs a) {
     }
 }
 
-// Per-path tile grid from the bounding box of all line end points, widened
-// by the DDA's worst-case overshoot: the crossing parameters are summed
-// sequentially (rasterizer.rs:110,114), so after k steps they may be off by
-// k * 2^-24, i.e. up to span^2 * 2^-24 pixels, plus one pixel of floor().
 __global__ void k_path_grid(const PathBox* box, PathGrid* grid, uint32_t* rows, int64_t n_paths, Counters* cnt) {
     int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (p >= n_paths) return;
     PathBox b = box[p];
     PathGrid g;
-    g.tx0 = g.ty0 = g.w = g.h = 0;
-    if (b.flags & BOX_LINES) {
-        if (b.flags & BOX_FULL) {
-            g.tx0 = g.ty0 = -4096;
-            g.w = g.h = 8192;
-        } else {
-            int64_t span = std::max<int64_t>((int64_t)b.maxx - b.minx, (int64_t)b.maxy - b.miny) + 2;
-            int64_t margin = 2 + (int64_t)ceil((double)span * (double)span * 1.2e-7);
-            int64_t px0 = std::max<int64_t>((int64_t)b.minx - margin, -32768);
-            int64_t py0 = std::max<int64_t>((int64_t)b.miny - margin, -32768);
-            int64_t px1 = std::min<int64_t>((int64_t)b.maxx + margin, 32767);
-            int64_t py1 = std::min<int64_t>((int64_t)b.maxy + margin, 32767);
-            g.tx0 = (int32_t)(px0 >> 3);
-            g.ty0 = (int32_t)(py0 >> 3);
-            g.w = (int32_t)((px1 >> 3) - g.tx0 + 1);
-            g.h = (int32_t)((py1 >> 3) - g.ty0 + 1);
-        }
-        atomicMax(&cnt->max_w, (uint32_t)g.w);
-    }
+    grid_from_box((b.flags & BOX_LINES) != 0, (b.flags & BOX_FULL) != 0, b.minx, b.miny, b.maxx, b.maxy, &g.tx0,
+                  &g.ty0, &g.w, &g.h);
+    if (b.flags & BOX_LINES) atomicMax(&cnt->max_w, (uint32_t)g.w);
     g.row_base = 0;
     grid[p] = g;
     rows[p] = (uint32_t)g.h + 1;  // + one row that carries the empty-path tile
@@ -398,58 +270,6 @@ struct StoreRowBase {
     PathGrid* g;
     __device__ void operator()(int64_t i, uint32_t excl, uint32_t) const { g[i].row_base = excl; }
 };
-
-// ------------------------------------------------------ (b) DDA binning
-//
-// Walks Rasterizer::line_to (rasterizer.rs:72-140) for one line and reports
-// tile runs (maximal runs of consecutive increments in one tile) and
-// tile-row crossings (TileIncrement, :123-131).  A run record holds the DDA
-// state one step before the run's first increment (or "line start"), which
-// is enough to replay the run bit-exactly later.
-struct RunRec {
-    uint32_t line;  // bit 31: run starts at the line's first increment
-    uint32_t xy;    // x | y << 16 (i16) of the state before the entering step
-    float row_t1, col_t1;
-};
-
-// Steps per line before the walk gives up: a line spanning the whole i16
-// pixel range needs ~2^17; more only arise from coordinates so large that
-// the reference would exhaust memory pushing increments.
-constexpr uint32_t WALK_LIMIT = 1u << 20;
-
-template <class Sink>
-__device__ inline void walk_line(V2 last, V2 point, Sink& sink) {
-    if (veq(point, last)) return;
-    Dda d;
-    dda_init(d, last, point);
-    int16_t ty_prev = div8(d.y);
-    int16_t cur_tx = div8(d.x), cur_ty = ty_prev;
-    sink.run(cur_tx, cur_ty, true, d);
-    for (uint32_t steps = 0;; steps++) {
-        if (steps >= WALK_LIMIT) {
-            sink.limit();
-            break;
-        }
-        Dda before = d;
-        bool term = dda_step(d);
-        if (term) {
-            d.x = f2i16(floorf(point.x));
-            d.y = f2i16(floorf(point.y));
-        }
-        int16_t ty = div8(d.y);
-        if (ty != ty_prev) {
-            sink.crossing(div8(d.x), ty_prev < ty ? ty_prev : ty, (int8_t)(int16_t)(ty - ty_prev));
-            ty_prev = ty;
-        }
-        if (term) break;
-        int16_t tx = div8(d.x);
-        if (tx != cur_tx || ty != cur_ty) {
-            cur_tx = tx;
-            cur_ty = ty;
-            sink.run(tx, ty, false, before);
-        }
-    }
-}
 
 struct CountSink {
     uint32_t runs = 0, entries = 0;
@@ -657,11 +477,14 @@ struct TileArgs {
 };
 
 constexpr int REPLAY_LIMIT = 64;  // a tile run has at most 15 increments
+constexpr int TG = 32;             // tiles per chunk (one lane each)
+constexpr int TSTR = 65;           // accumulator stride per tile (spreads LDS banks)
+constexpr int ROWS_PER_WAVE = 8;   // tile rows per wave
+constexpr int STAGE = 256;         // run records staged in LDS per pass
 
-// Replays one tile run into this lane's accumulator column.
-__device__ inline void replay_run(const RunRec& r, const float4* lines, int16_t tx, int16_t ty, float* acc_a,
-                                  float* acc_h, int lane, Counters* cnt) {
-    float4 L = lines[r.line & 0x7fffffffu];
+// Replays one tile run into the accumulator of tile slot `t`.
+__device__ inline void replay_run(const RunRec& r, const float4& L, int16_t tx, int16_t ty, float* acc_a,
+                                  float* acc_h, int t, Counters* cnt) {
     Dda d;
     V2 last = mk(L.x, L.y), point = mk(L.z, L.w);
     if (r.line >> 31) {
@@ -679,119 +502,200 @@ __device__ inline void replay_run(const RunRec& r, const float4* lines, int16_t 
         float area, height;
         dda_increment(d, area, height);
         int pix = ((d.y & 7) << 3) | (d.x & 7);
-        acc_a[pix * WAVE + lane] += area;
-        acc_h[pix * WAVE + lane] += height;
+        acc_a[t * TSTR + pix] += area;
+        acc_h[t * TSTR + pix] += height;
         if (dda_step(d)) return;
         if (div8(d.x) != tx || div8(d.y) != ty) return;
     }
     atomicOr(&cnt->error, (uint32_t)ERR_REPLAY);
 }
 
+// Inclusive scan restarting at every lane with head set.
+__device__ inline int32_t seg_incl_scan(int32_t v, bool head) {
+    const int lane = lane_id();
+    int f = head ? 1 : 0;
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) {
+        int32_t ov = __shfl_up(v, d, WAVE);
+        int of = __shfl_up(f, d, WAVE);
+        if (lane >= d && !f) v = ov + v;
+        if (lane >= d) f |= of;
+    }
+    return v;
+}
+
+// One wave per block of 8 consecutive tile rows; each chunk of up to 32
+// tiles (groups) is handled one lane per tile.  The count pass only counts
+// tiles and spans per row; the emit pass also accumulates, carries along
+// rows and writes.
 template <bool EMIT>
 __global__ __launch_bounds__(WAVE) void k_tiles(TileArgs a) {
-    __shared__ float acc_a[EMIT ? 64 * WAVE : 1];
-    __shared__ float acc_h[EMIT ? 64 * WAVE : 1];
-    __shared__ float cin[EMIT ? WAVE * 8 : 1];
+    __shared__ float acc_a[EMIT ? TG * TSTR : 1];
+    __shared__ float acc_h[EMIT ? TG * TSTR : 1];
+    __shared__ float cin[EMIT ? TG * 8 : 1];
+    __shared__ float4 st_line[EMIT ? STAGE : 1];
+    __shared__ RunRec st_rec[EMIT ? STAGE : 1];
+    __shared__ uint64_t slot_tiles[ROWS_PER_WAVE];
+    // per row slot: running winding, tiles, spans, pending tile (for spans)
+    __shared__ int32_t r_w[ROWS_PER_WAVE], r_pend_tx[ROWS_PER_WAVE], r_pend_w[ROWS_PER_WAVE];
+    __shared__ uint32_t r_nt[ROWS_PER_WAVE], r_ns[ROWS_PER_WAVE], r_pend[ROWS_PER_WAVE];
     const int lane = threadIdx.x;
     const uint64_t lt = lanemask_lt();
     const uint64_t le = lane == 63 ? ~0ull : ((2ull << lane) - 1);
     const uint32_t xmask = (1u << a.xbits) - 1;
-    for (int64_t row = blockIdx.x; row < a.n_rows; row += gridDim.x) {
-        const uint32_t p = a.row_path[row];
-        const PathGrid G = a.grid[p];
-        const int64_t local = row - (int64_t)G.row_base;
-        if (local >= G.h) continue;  // the empty-path row
-        const uint32_t gb = a.row_gbegin[row], ge = a.row_gend[row];
-        if (gb >= ge) {
-            if (!EMIT && lane == 0) a.row_cnt[row] = 0;
-            continue;
+    const int64_t n_blocks = (a.n_rows + ROWS_PER_WAVE - 1) / ROWS_PER_WAVE;
+    for (int64_t blk = blockIdx.x; blk < n_blocks; blk += gridDim.x) {
+        const int64_t R0 = blk * ROWS_PER_WAVE;
+        // lanes 0..7 describe the block's rows
+        uint32_t gb = 0xffffffffu, ge = 0;
+        bool real_row = false;
+        if (lane < ROWS_PER_WAVE && R0 + lane < a.n_rows) {
+            const int64_t row = R0 + lane;
+            const uint32_t p = a.row_path[row];
+            const PathGrid G = a.grid[p];
+            real_row = row - (int64_t)G.row_base < G.h;  // not the empty-path row
+            if (real_row) {
+                uint32_t b = a.row_gbegin[row], e = a.row_gend[row];
+                if (e > b) {
+                    gb = b;
+                    ge = e;
+                }
+                r_w[lane] = a.row_wb[row];
+            }
+            r_nt[lane] = 0;
+            r_ns[lane] = 0;
+            r_pend[lane] = 0;
         }
-        const int32_t ty = G.ty0 + (int32_t)local;
-        int32_t winding = a.row_wb[row];
-        uint32_t n_tiles = 0, n_spans = 0;
-        bool pend = false;
-        int32_t pend_tx = 0, pend_w = 0;
-        float carry = 0.0f;
-        uint64_t off = EMIT ? a.row_off[row] : 0;
-        const uint32_t tile_base = (uint32_t)(off >> 32), span_base = (uint32_t)off;
-        for (uint32_t c = gb; c < ge; c += WAVE) {
+        uint32_t G0 = gb, G1 = ge;
+#pragma unroll
+        for (int d = 1; d < ROWS_PER_WAVE; d <<= 1) {
+            G0 = min(G0, (uint32_t)__shfl_xor(G0, d, WAVE));
+            G1 = max(G1, (uint32_t)__shfl_xor(G1, d, WAVE));
+        }
+        G0 = __shfl(G0, 0, WAVE);
+        G1 = __shfl(G1, 0, WAVE);
+        float carry = 0.0f;  // lane = slot * 8 + pixel row
+        __syncthreads();
+        for (uint32_t c = G0; c < G1 && G0 < G1; c += TG) {
             const uint32_t g = c + lane;
-            const bool valid = g < ge;
+            const bool valid = lane < TG && g < G1;
             const uint32_t key = valid ? a.gkey[g] : 0u;
-            const int32_t tx = G.tx0 + (int32_t)(key & xmask);
+            const int64_t row = (int64_t)(key >> a.xbits);
+            const int slot = valid ? (int)(row - R0) : ROWS_PER_WAVE;
+            int32_t tx = 0, ty = 0;
+            uint32_t row_tiles = 0, row_spans = 0;
+            if (valid) {
+                const PathGrid G = a.grid[a.row_path[row]];
+                tx = G.tx0 + (int32_t)(key & xmask);
+                ty = G.ty0 + (int32_t)(row - (int64_t)G.row_base);
+                if (EMIT) {
+                    uint64_t off = a.row_off[row];
+                    row_tiles = (uint32_t)(off >> 32);
+                    row_spans = (uint32_t)off;
+                }
+            }
             const bool hr = valid && a.ghasrun[g];
             const int32_t sg = valid ? a.gsign[g] : 0;
-            const int32_t incl = wave_incl_scan(sg, OpAdd());
-            const int32_t wend = winding + incl;  // winding after this key's crossings
-            winding += __shfl(incl, WAVE - 1, WAVE);
+            const int prev_slot = __shfl_up(slot, 1, WAVE);
+            const bool head = lane == 0 || prev_slot != slot;
+            // lanes of my row slot
+            uint64_t same = __ballot(valid);
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                const bool bit = (slot >> b) & 1;
+                const uint64_t m = __ballot(bit);
+                same &= bit ? m : ~m;
+            }
             const uint64_t tiles = __ballot(hr);
-            // spans between consecutive tiles of this chunk (rasterizer.rs:252-264)
-            const uint64_t above = tiles & ~le;
+            const int32_t wend = (valid ? r_w[slot] : 0) + seg_incl_scan(sg, head);
+            const uint64_t my_tiles = tiles & same;
+            const uint32_t rank = (valid ? r_nt[slot] : 0u) + (uint32_t)__popcll(my_tiles & lt);
+            // spans to the next tile of the same row (rasterizer.rs:252-264)
+            const uint64_t above = my_tiles & ~le;
             const int nb = above ? (__ffsll((long long)above) - 1) : lane;
             const int32_t btx = __shfl(tx, nb, WAVE);
             const bool span_here = hr && above && btx > tx + 1 && wend != 0;
-            // span closing the previous chunk's last tile
-            const int ft = tiles ? (__ffsll((long long)tiles) - 1) : 0;
-            const int32_t ftx = __shfl(tx, ft, WAVE);
+            // the first tile of a row in this chunk closes that row's pending tile
+            const bool first_tile = hr && (my_tiles & lt) == 0;
             bool pend_span = false;
-            if (pend && tiles) {
-                pend_span = ftx > pend_tx + 1 && pend_w != 0;
-                pend = false;
+            int32_t ptx = 0;
+            if (first_tile && r_pend[slot]) {
+                ptx = r_pend_tx[slot];
+                pend_span = tx > ptx + 1 && r_pend_w[slot] != 0;
             }
             const uint64_t spans = __ballot(span_here);
+            const uint64_t pends = __ballot(pend_span);
+            const uint32_t my_spans_before = (uint32_t)__popcll(spans & same & lt) + (uint32_t)__popcll(pends & same);
+            const uint32_t span_base = valid ? r_ns[slot] : 0u;
             if (EMIT) {
-                if (pend_span && lane == 0) {
-                    uint32_t si = span_base + n_spans;
-                    a.span_xy[2 * si] = (int16_t)((pend_tx + 1) * TILE);
+                if (pend_span) {
+                    const uint32_t si = row_spans + span_base;
+                    a.span_xy[2 * si] = (int16_t)((ptx + 1) * TILE);
                     a.span_xy[2 * si + 1] = (int16_t)(ty * TILE);
-                    a.span_w[si] = (uint16_t)((uint16_t)(ftx - pend_tx - 1) * (uint16_t)TILE);
+                    a.span_w[si] = (uint16_t)((uint16_t)(tx - ptx - 1) * (uint16_t)TILE);
                 }
                 if (span_here) {
-                    uint32_t si = span_base + n_spans + (pend_span ? 1u : 0u) + (uint32_t)__popcll(spans & lt);
+                    const uint32_t si = row_spans + span_base + my_spans_before;
                     a.span_xy[2 * si] = (int16_t)((tx + 1) * TILE);
                     a.span_xy[2 * si + 1] = (int16_t)(ty * TILE);
                     a.span_w[si] = (uint16_t)((uint16_t)(btx - tx - 1) * (uint16_t)TILE);
                 }
             }
-            n_spans += (pend_span ? 1u : 0u) + (uint32_t)__popcll(spans);
-            if (tiles) {
-                const int lt_lane = 63 - __clzll((long long)tiles);
-                pend = true;
-                pend_tx = __shfl(tx, lt_lane, WAVE);
-                pend_w = __shfl(wend, lt_lane, WAVE);
-            }
             if (EMIT) {
-                // (c) accumulation, one lane per tile, runs in emission order
-                for (int pix = 0; pix < 64; pix++) {
-                    acc_a[pix * WAVE + lane] = 0.0f;
-                    acc_h[pix * WAVE + lane] = 0.0f;
+                // ---- (c) accumulate: stage run records, replay one lane per tile
+                for (int k = lane; k < TG * TSTR; k += WAVE) {
+                    acc_a[k] = 0.0f;
+                    acc_h[k] = 0.0f;
                 }
-                if (hr) {
-                    const uint32_t e0 = a.gbegin[g], e1 = a.gbegin[g + 1];
-                    const int16_t ttx = (int16_t)tx, tty = (int16_t)ty;
-                    for (uint32_t e = e0; e < e1; e++) {
-                        uint32_t v = a.vals[e];
-                        if (v >> 31) continue;
-                        RunRec r = a.recs[v];
-                        replay_run(r, a.lines, ttx, tty, acc_a, acc_h, lane, a.cnt);
+                const uint32_t last_g = min(c + (uint32_t)TG, G1);
+                const uint32_t E0 = a.gbegin[c], E1 = a.gbegin[last_g];
+                const uint32_t my_e0 = hr ? a.gbegin[g] : 0u, my_e1 = hr ? a.gbegin[g + 1] : 0u;
+                for (uint32_t s0 = E0; s0 < E1; s0 += STAGE) {
+                    const uint32_t n = min((uint32_t)STAGE, E1 - s0);
+                    __syncthreads();
+                    for (uint32_t k = lane; k < n; k += WAVE) {
+                        const uint32_t v = a.vals[s0 + k];
+                        RunRec r;
+                        if (v >> 31) {
+                            r.line = 0xffffffffu;
+                        } else {
+                            r = a.recs[v];
+                            st_line[k] = a.lines[r.line & 0x7fffffffu];
+                        }
+                        st_rec[k] = r;
+                    }
+                    __syncthreads();
+                    if (hr) {
+                        const uint32_t e_lo = max(my_e0, s0), e_hi = min(my_e1, s0 + n);
+                        for (uint32_t e = e_lo; e < e_hi; e++) {
+                            const RunRec r = st_rec[e - s0];
+                            if (r.line == 0xffffffffu) continue;
+                            replay_run(r, st_line[e - s0], (int16_t)tx, (int16_t)ty, acc_a, acc_h, lane, a.cnt);
+                        }
                     }
                 }
-                __syncthreads();
-                // (d) carry along the tile row, one lane per pixel row, sequential
-                if (lane < 8) {
-                    uint64_t tm = tiles;
-                    while (tm) {
-                        int t = __ffsll((long long)tm) - 1;
-                        tm &= tm - 1;
-                        cin[t * 8 + lane] = carry;
+                // ---- (d) carry along each row: lane = slot * 8 + pixel row
 #pragma unroll
-                        for (int x = 0; x < 8; x++) carry += acc_h[(lane * 8 + x) * WAVE + t];
+                for (int rs = 0; rs < ROWS_PER_WAVE; rs++) {  // tiles of each row slot in this chunk
+                    const uint64_t m = __ballot(hr && slot == rs);
+                    if (lane == rs) slot_tiles[rs] = m;
+                }
+                __syncthreads();
+                {
+                    const int cs = lane >> 3, y = lane & 7;
+                    uint64_t m = slot_tiles[cs];
+                    while (m) {
+                        const int t = __ffsll((long long)m) - 1;
+                        m &= m - 1;
+                        cin[t * 8 + y] = carry;
+#pragma unroll
+                        for (int x = 0; x < 8; x++) carry += acc_h[t * TSTR + y * 8 + x];
                     }
                 }
                 __syncthreads();
-                // (e) quantise and write (rasterizer.rs:231-241)
+                // ---- (e) quantise and write (rasterizer.rs:231-241)
                 if (hr) {
-                    uint32_t ti = tile_base + n_tiles + (uint32_t)__popcll(tiles & lt);
+                    const uint32_t ti = row_tiles + rank;
                     uint32_t words[16];
 #pragma unroll
                     for (int y = 0; y < 8; y++) {
@@ -799,12 +703,12 @@ __global__ __launch_bounds__(WAVE) void k_tiles(TileArgs a) {
                         uint32_t w0 = 0, w1 = 0;
 #pragma unroll
                         for (int x = 0; x < 8; x++) {
-                            int pix = y * 8 + x;
-                            float v = fminf(fabsf(accum + acc_a[pix * WAVE + lane]) * 256.0f, 255.0f);
+                            const int k = lane * TSTR + y * 8 + x;
+                            float v = fminf(fabsf(accum + acc_a[k]) * 256.0f, 255.0f);
                             uint32_t b = (uint32_t)v;
                             if (x < 4) w0 |= b << (8 * x);
                             else w1 |= b << (8 * (x - 4));
-                            accum += acc_h[pix * WAVE + lane];
+                            accum += acc_h[k];
                         }
                         words[2 * y] = w0;
                         words[2 * y + 1] = w1;
@@ -817,11 +721,29 @@ __global__ __launch_bounds__(WAVE) void k_tiles(TileArgs a) {
                     a.tile_xy[2 * ti] = (int16_t)(tx * TILE);
                     a.tile_xy[2 * ti + 1] = (int16_t)(ty * TILE);
                 }
-                __syncthreads();
             }
-            n_tiles += (uint32_t)__popcll(tiles);
+            __syncthreads();
+            // per-row state after this chunk: the last lane of each row segment
+            const int next_slot = __shfl_down(slot, 1, WAVE);
+            const bool tail = valid && (lane == WAVE - 1 || next_slot != slot || !(lane + 1 < TG && c + lane + 1 < G1));
+            const int last = my_tiles ? 63 - __clzll((long long)my_tiles) : lane;
+            const int32_t ltx = __shfl(tx, last, WAVE);
+            const int32_t lw = __shfl(wend, last, WAVE);
+            if (tail) {
+                r_w[slot] = wend;
+                r_nt[slot] += (uint32_t)__popcll(my_tiles);
+                r_ns[slot] += (uint32_t)__popcll(spans & same) + (uint32_t)__popcll(pends & same);
+                if (my_tiles) {
+                    r_pend[slot] = 1;
+                    r_pend_tx[slot] = ltx;
+                    r_pend_w[slot] = lw;
+                }
+            }
+            __syncthreads();
         }
-        if (!EMIT && lane == 0) a.row_cnt[row] = ((uint64_t)n_tiles << 32) | n_spans;
+        if (!EMIT && lane < ROWS_PER_WAVE && R0 + lane < a.n_rows && real_row)
+            a.row_cnt[R0 + lane] = ((uint64_t)r_nt[lane] << 32) | r_ns[lane];
+        __syncthreads();
     }
 }
 
@@ -850,11 +772,7 @@ __global__ void k_path_out(const PathGrid* grid, int64_t n_paths, const uint64_t
 
 // ------------------------------------------------------------- host driver
 
-#define HIPCHK(x)                                   \
-    do {                                            \
-        hipError_t e_ = (x);                        \
-        if (e_ != hipSuccess) return map_hip_error(e_); \
-    } while (0)
+#define HIPCHK(x) HIPCHK_RET(x)
 
 static inline uint32_t grid_for(int64_t n, int bs) { return (uint32_t)((n + bs - 1) / bs); }
 
@@ -1061,7 +979,7 @@ int run_plain(Context* ctx, const SegInput& in, OutputDesc& out, hipStream_t s) 
     ta.span_xy = out.span_xy;
     ta.span_w = out.span_w;
     ta.cnt = cnt;
-    const uint32_t tile_blocks = (uint32_t)std::min<int64_t>(ROWS, 1 << 20);
+    const uint32_t tile_blocks = (uint32_t)std::min<int64_t>((ROWS + ROWS_PER_WAVE - 1) / ROWS_PER_WAVE, 1 << 20);
     hipLaunchKernelGGL(k_tiles<false>, dim3(tile_blocks), dim3(WAVE), 0, s, ta);
     HIPCHK(launch_scan<uint64_t>(ctx, ROWS, LoadU64{ta.row_cnt}, StoreU64{ctx->row_off.as<uint64_t>()}, OpAdd(),
                                  0ull, &tot[4], s));

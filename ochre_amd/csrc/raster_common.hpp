@@ -1,0 +1,191 @@
+// Device code shared by the plain and the batched pipelines: command ->
+// line enumeration (path.rs:41-109, rasterizer.rs:61-69,145-165,181-183),
+// the per-path tile grid, and the DDA walk (rasterizer.rs:72-140).
+#pragma once
+
+#include "device_common.hpp"
+
+namespace ochre {
+
+// Resolved command: its transformed points and the rasterizer state
+// (`last`, `first`) it starts from (rasterizer.rs:145-165).
+struct Cmd {
+    int verb;
+    int32_t path;
+    bool last_in_path;
+    bool bad;
+    V2 pt[3];
+    float w;
+    V2 start, first;
+};
+
+// Enumerates the lines a command feeds to line_to (rasterizer.rs:61-69,
+// 145-157 and the finish-time close, :181-183), in order.
+template <class Emit>
+__device__ inline bool command_lines(const Cmd& c, Emit&& emit) {
+    bool ok = true;
+    V2 last = c.start;
+    V2 first = c.first;
+    switch (c.verb) {
+    case MOVE:
+        if (!veq(last, first)) emit(last, first);
+        last = c.pt[0];
+        first = c.pt[0];
+        break;
+    case LINE:
+        emit(last, c.pt[0]);
+        last = c.pt[0];
+        break;
+    case QUAD: {
+        float dt = quad_dt(last, c.pt[0], c.pt[1]);
+        float t = 0.0f;
+        uint32_t n = 0;
+        V2 prev = last;
+        while (t < 1.0f) {
+            t = fminf(t + dt, 1.0f);
+            V2 q = quad_at(t, last, c.pt[0], c.pt[1]);
+            emit(prev, q);
+            prev = q;
+            if (++n >= FLATTEN_LIMIT) {
+                ok = false;
+                break;
+            }
+        }
+        last = prev;
+        break;
+    }
+    case CUBIC: {
+        float dt = cubic_dt(last, c.pt[0], c.pt[1], c.pt[2]);
+        float t = 0.0f;
+        uint32_t n = 0;
+        V2 prev = last;
+        while (t < 1.0f) {
+            t = fminf(t + dt, 1.0f);
+            V2 q = cubic_at(t, last, c.pt[0], c.pt[1], c.pt[2]);
+            emit(prev, q);
+            prev = q;
+            if (++n >= FLATTEN_LIMIT) {
+                ok = false;
+                break;
+            }
+        }
+        last = prev;
+        break;
+    }
+    case CONIC: {
+        V2 prev = last;
+        ok = conic_walk(last, c.pt[0], c.pt[1], c.w, [&](V2 q) {
+            emit(prev, q);
+            prev = q;
+        });
+        last = prev;
+        break;
+    }
+    default:  // CLOSE: a no-op in the rasterizer (rasterizer.rs:154)
+        break;
+    }
+    if (c.last_in_path && !veq(last, first)) emit(last, first);
+    return ok;
+}
+
+constexpr float HUGE_COORD = 30000.0f;
+enum BoxFlags : uint32_t { BOX_FULL = 1u, BOX_LINES = 2u };
+
+struct BoxAcc {
+    float minx = INFINITY, miny = INFINITY, maxx = -INFINITY, maxy = -INFINITY;
+    bool full = false;
+    __device__ void add(V2 p) {
+        if (!(fabsf(p.x) <= HUGE_COORD) || !(fabsf(p.y) <= HUGE_COORD)) {
+            full = true;
+            return;
+        }
+        minx = fminf(minx, p.x);
+        miny = fminf(miny, p.y);
+        maxx = fmaxf(maxx, p.x);
+        maxy = fmaxf(maxy, p.y);
+    }
+};
+
+// ------------------------------------------------------ (b) DDA binning
+//
+// Walks Rasterizer::line_to (rasterizer.rs:72-140) for one line and reports
+// tile runs (maximal runs of consecutive increments in one tile) and
+// tile-row crossings (TileIncrement, :123-131).  A run record holds the DDA
+// state one step before the run's first increment (or "line start"), which
+// is enough to replay the run bit-exactly later.
+struct RunRec {
+    uint32_t line;  // bit 31: run starts at the line's first increment
+    uint32_t xy;    // x | y << 16 (i16) of the state before the entering step
+    float row_t1, col_t1;
+};
+
+// Steps per line before the walk gives up: a line spanning the whole i16
+// pixel range needs ~2^17; more only arise from coordinates so large that
+// the reference would exhaust memory pushing increments.
+constexpr uint32_t WALK_LIMIT = 1u << 20;
+
+template <class Sink>
+__device__ inline void walk_line(V2 last, V2 point, Sink& sink) {
+    if (veq(point, last)) return;
+    Dda d;
+    dda_init(d, last, point);
+    int16_t ty_prev = div8(d.y);
+    int16_t cur_tx = div8(d.x), cur_ty = ty_prev;
+    sink.run(cur_tx, cur_ty, true, d);
+    for (uint32_t steps = 0;; steps++) {
+        if (steps >= WALK_LIMIT) {
+            sink.limit();
+            break;
+        }
+        Dda before = d;
+        bool term = dda_step(d);
+        if (term) {
+            d.x = f2i16(floorf(point.x));
+            d.y = f2i16(floorf(point.y));
+        }
+        int16_t ty = div8(d.y);
+        if (ty != ty_prev) {
+            sink.crossing(div8(d.x), ty_prev < ty ? ty_prev : ty, (int8_t)(int16_t)(ty - ty_prev));
+            ty_prev = ty;
+        }
+        if (term) break;
+        int16_t tx = div8(d.x);
+        if (tx != cur_tx || ty != cur_ty) {
+            cur_tx = tx;
+            cur_ty = ty;
+            sink.run(tx, ty, false, before);
+        }
+    }
+}
+
+
+// Tile grid of a path from the bounding box (pixel floors) of all its line
+// end points, widened by the DDA's worst-case overshoot: the crossing
+// parameters are summed sequentially (rasterizer.rs:110,114), so after k
+// steps they may be off by k * 2^-24, i.e. up to span^2 * 2^-24 pixels, plus
+// one pixel of floor().  Paths with huge or non-finite coordinates get the
+// whole i16 range (the DDA's coordinates wrap in i16).
+__device__ inline void grid_from_box(bool has_lines, bool full, int32_t minx, int32_t miny, int32_t maxx,
+                                     int32_t maxy, int32_t* tx0, int32_t* ty0, int32_t* w, int32_t* h) {
+    *tx0 = *ty0 = *w = *h = 0;
+    if (!has_lines) return;
+    if (full) {
+        *tx0 = *ty0 = -4096;
+        *w = *h = 8192;
+        return;
+    }
+    int64_t span = ((int64_t)maxx - minx > (int64_t)maxy - miny ? (int64_t)maxx - minx : (int64_t)maxy - miny) + 2;
+    int64_t margin = 2 + (int64_t)ceil((double)span * (double)span * 1.2e-7);
+    int64_t px0 = (int64_t)minx - margin, py0 = (int64_t)miny - margin;
+    int64_t px1 = (int64_t)maxx + margin, py1 = (int64_t)maxy + margin;
+    if (px0 < -32768) px0 = -32768;
+    if (py0 < -32768) py0 = -32768;
+    if (px1 > 32767) px1 = 32767;
+    if (py1 > 32767) py1 = 32767;
+    *tx0 = (int32_t)(px0 >> 3);
+    *ty0 = (int32_t)(py0 >> 3);
+    *w = (int32_t)((px1 >> 3) - *tx0 + 1);
+    *h = (int32_t)((py1 >> 3) - *ty0 + 1);
+}
+
+}  // namespace ochre

@@ -1,10 +1,4 @@
-// Device-wide single-pass exclusive scan (decoupled look-back) and wave/block
-// scan helpers for wave64.
-//
-// Look-back state is tagged with an epoch read from device memory at kernel
-// start; the last block to finish bumps the epoch and resets the block
-// dispenser, so the state never needs clearing (no memset nodes in captured
-// graphs, no host round trip).
+// Wave/block scan helpers for wave64 and a device-wide exclusive scan.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -80,45 +74,17 @@ __device__ inline T block_excl_scan(T v, Op op, T identity, T* smem, T* total) {
     return op(wave_prefix, excl);
 }
 
-// Look-back scratch for one scan call site.  Sized for max_blocks blocks.
-template <typename T>
-struct ScanState {
-    uint32_t* flags;     // per block: (epoch << 2) | status, status 1 = aggregate, 2 = inclusive
-    T* aggregate;
-    T* inclusive;
-    uint32_t* ctl;       // [0] block dispenser, [1] finished blocks, [2] epoch
-};
-
-template <typename T>
-__device__ inline void store_release(uint32_t* flag, uint32_t v) {
-    __threadfence();
-    __hip_atomic_store(flag, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
+// Device-wide exclusive scan as reduce-then-scan: (1) every block reduces its
+// tile, (2) one block scans the tile sums, (3) every block rescans its tile
+// from its prefix.  All three launches are fully parallel; on a multi-XCD
+// part a single-pass look-back chain would poll flags across dies (one
+// Infinity Fabric round trip per link of the chain).
 constexpr int SCAN_NT = 256;
-constexpr int SCAN_ITEMS = 8;
+constexpr int SCAN_ITEMS = 16;
 constexpr int SCAN_TILE = SCAN_NT * SCAN_ITEMS;
 
-// Exclusive scan: store(i, exclusive_prefix_i, value_i) for i in [0, n).
-// Load: T load(int64_t i) for i < n.  The final total is written to *total
-// if total != nullptr.
-template <typename T, typename Op, typename Load, typename Store>
-__global__ __launch_bounds__(SCAN_NT) void k_scan(Load load, Store store, int64_t n, ScanState<T> st,
-                                                  Op op, T identity, T* total) {
-    __shared__ uint32_t s_bid, s_epoch;
-    __shared__ T s_red[SCAN_NT / WAVE];
-    __shared__ T s_prefix;
-    __shared__ int s_last;
-    const int tid = threadIdx.x;
-    if (tid == 0) {
-        s_bid = atomicAdd(&st.ctl[0], 1u);
-        s_epoch = __hip_atomic_load(&st.ctl[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    __syncthreads();
-    const uint32_t bid = s_bid;
-    const uint32_t epoch = s_epoch;
-    const int64_t base = (int64_t)bid * SCAN_TILE + (int64_t)tid * SCAN_ITEMS;
-    T v[SCAN_ITEMS];
+template <typename T, typename Op, typename Load>
+__device__ inline T tile_load_reduce(Load load, int64_t base, int64_t n, Op op, T identity, T* v) {
     T acc = identity;
 #pragma unroll
     for (int j = 0; j < SCAN_ITEMS; j++) {
@@ -126,61 +92,82 @@ __global__ __launch_bounds__(SCAN_NT) void k_scan(Load load, Store store, int64_
         v[j] = i < n ? load(i) : identity;
         acc = op(acc, v[j]);
     }
-    T block_total;
-    T excl = block_excl_scan<SCAN_NT>(acc, op, identity, s_red, &block_total);
-    if (tid == 0) {
-        T prefix = identity;
-        const uint32_t tag = epoch << 2;
-        if (bid == 0) {
-            st.inclusive[0] = block_total;
-            store_release<T>(&st.flags[0], tag | 2u);
-        } else {
-            st.aggregate[bid] = block_total;
-            store_release<T>(&st.flags[bid], tag | 1u);
-            int64_t p = (int64_t)bid - 1;
-            for (;;) {
-                uint32_t f = __hip_atomic_load(&st.flags[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if ((f & ~3u) != tag || (f & 3u) == 0) {
-                    __builtin_amdgcn_s_sleep(1);
-                    continue;
-                }
-                __threadfence();
-                if ((f & 3u) == 2u) {
-                    prefix = op(st.inclusive[p], prefix);
-                    break;
-                }
-                prefix = op(st.aggregate[p], prefix);
-                p--;
-            }
-            st.inclusive[bid] = op(prefix, block_total);
-            store_release<T>(&st.flags[bid], tag | 2u);
+    return acc;
+}
+
+template <typename T, typename Op, typename Load>
+__global__ __launch_bounds__(SCAN_NT) void k_scan_reduce(Load load, int64_t n, T* partials, Op op, T identity) {
+    __shared__ T s_red[SCAN_NT / WAVE];
+    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+    T v[SCAN_ITEMS];
+    T acc = tile_load_reduce<T>(load, base, n, op, identity, v);
+    T total;
+    (void)block_excl_scan<SCAN_NT>(acc, op, identity, s_red, &total);
+    if (threadIdx.x == 0) partials[blockIdx.x] = total;
+}
+
+// One block: exclusive scan of partials[0, nb) in place; *total = sum.
+template <typename T, typename Op>
+__global__ __launch_bounds__(SCAN_NT) void k_scan_partials(T* partials, int64_t nb, Op op, T identity, T* total) {
+    __shared__ T s_red[SCAN_NT / WAVE];
+    T carry = identity;
+    for (int64_t b0 = 0; b0 < nb; b0 += SCAN_TILE) {
+        const int64_t base = b0 + (int64_t)threadIdx.x * SCAN_ITEMS;
+        T v[SCAN_ITEMS];
+        T acc = identity;
+#pragma unroll
+        for (int j = 0; j < SCAN_ITEMS; j++) {
+            int64_t i = base + j;
+            v[j] = i < nb ? partials[i] : identity;
+            acc = op(acc, v[j]);
         }
-        s_prefix = prefix;
+        T chunk;
+        T r = op(carry, block_excl_scan<SCAN_NT>(acc, op, identity, s_red, &chunk));
+#pragma unroll
+        for (int j = 0; j < SCAN_ITEMS; j++) {
+            int64_t i = base + j;
+            if (i < nb) partials[i] = r;
+            r = op(r, v[j]);
+        }
+        carry = op(carry, chunk);
     }
-    __syncthreads();
-    T r = op(s_prefix, excl);
+    if (threadIdx.x == 0 && total != nullptr) *total = carry;
+}
+
+template <typename T, typename Op, typename Load, typename Store>
+__global__ __launch_bounds__(SCAN_NT) void k_scan_down(Load load, Store store, int64_t n, const T* partials, Op op,
+                                                       T identity) {
+    __shared__ T s_red[SCAN_NT / WAVE];
+    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+    T v[SCAN_ITEMS];
+    T acc = tile_load_reduce<T>(load, base, n, op, identity, v);
+    T total;
+    T r = op(partials ? partials[blockIdx.x] : identity, block_excl_scan<SCAN_NT>(acc, op, identity, s_red, &total));
 #pragma unroll
     for (int j = 0; j < SCAN_ITEMS; j++) {
         int64_t i = base + j;
         if (i < n) store(i, r, v[j]);
         r = op(r, v[j]);
     }
-    const uint32_t nblocks = (uint32_t)((n + SCAN_TILE - 1) / SCAN_TILE);
-    if (total != nullptr && bid == nblocks - 1 && tid == SCAN_NT - 1) *total = r;
-    // The last block to finish retires this epoch.
-    __syncthreads();
-    if (tid == 0) {
-        __threadfence();
-        uint32_t done = atomicAdd(&st.ctl[1], 1u);
-        s_last = (done == nblocks - 1);
+}
+
+// Single-tile scan that also writes the total.
+template <typename T, typename Op, typename Load, typename Store>
+__global__ __launch_bounds__(SCAN_NT) void k_scan_single(Load load, Store store, int64_t n, Op op, T identity,
+                                                         T* total_out) {
+    __shared__ T s_red[SCAN_NT / WAVE];
+    const int64_t base = (int64_t)threadIdx.x * SCAN_ITEMS;
+    T v[SCAN_ITEMS];
+    T acc = tile_load_reduce<T>(load, base, n, op, identity, v);
+    T total;
+    T r = block_excl_scan<SCAN_NT>(acc, op, identity, s_red, &total);
+#pragma unroll
+    for (int j = 0; j < SCAN_ITEMS; j++) {
+        int64_t i = base + j;
+        if (i < n) store(i, r, v[j]);
+        r = op(r, v[j]);
     }
-    __syncthreads();
-    if (s_last && tid == 0) {
-        st.ctl[0] = 0;
-        st.ctl[1] = 0;
-        __hip_atomic_store(&st.ctl[2], epoch + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __threadfence();
-    }
+    if (threadIdx.x == 0 && total_out != nullptr) *total_out = total;
 }
 
 struct OpAdd {
@@ -188,9 +175,5 @@ struct OpAdd {
     __device__ T operator()(T a, T b) const { return a + b; }
 };
 
-template <typename T>
-inline uint32_t scan_blocks(int64_t n) {
-    return (uint32_t)((n + SCAN_TILE - 1) / SCAN_TILE);
-}
 
 }  // namespace ochre

@@ -1,4 +1,5 @@
-"""Parity of the plain five-kernel pipeline with the oracle, through the C ABI."""
+"""Parity of both device pipelines (plain five-kernel, batched persistent)
+with the oracle, through the C ABI."""
 import numpy as np
 import pytest
 
@@ -9,52 +10,59 @@ from parity import compare
 pytestmark = pytest.mark.gpu
 
 
-def run_plain(ctx, batch):
+MODES = ["plain", "batched"]
+
+
+def check(ctx, batch, mode, max_lsb=1, min_exact=None):
     from ochre_amd.device import run_to_host
-    return run_to_host(ctx, batch, "plain")
-
-
-def check(ctx, batch, max_lsb=1, min_exact=None):
-    print(f"[plain] {batch.name} paths={batch.n_paths}", flush=True)
+    print(f"[{mode}] {batch.name} paths={batch.n_paths}", flush=True)
     ref = oracle_ref.run_batch(batch)
-    got = run_plain(ctx, batch)
+    got = run_to_host(ctx, batch, mode)
     st = compare(ref, got, max_lsb=max_lsb)
-    print(f"[plain] {batch.name}: {st}", flush=True)
+    print(f"[{mode}] {batch.name}: {st}", flush=True)
     if min_exact is not None:
         assert st["bit_exact_fraction"] >= min_exact
     return st
 
 
-def test_readme_path(gpu_ctx):
-    st = check(gpu_ctx, W.readme_path(), max_lsb=0)
+@pytest.mark.parametrize("mode", MODES)
+def test_readme_path(gpu_ctx, mode):
+    st = check(gpu_ctx, W.readme_path(), mode, max_lsb=0)
     assert st["tiles"] == 100 and st["spans"] == 23
 
 
+@pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("seed", [0, 1, 2])
-def test_random_paths(gpu_ctx, seed):
-    check(gpu_ctx, W.random_paths(300, seed), max_lsb=0)
+def test_random_paths(gpu_ctx, seed, mode):
+    check(gpu_ctx, W.random_paths(300, seed), mode, max_lsb=0)
 
 
-def test_random_paths_identity_integer(gpu_ctx):
+@pytest.mark.parametrize("mode", MODES)
+def test_random_paths_identity_integer(gpu_ctx, mode):
     # integer and half-integer coordinates exercise DDA ties and zero-length steps
-    check(gpu_ctx, W.random_paths(300, 7, integer_prob=1.0, xform="identity"), max_lsb=0)
+    check(gpu_ctx, W.random_paths(300, 7, integer_prob=1.0, xform="identity"), mode, max_lsb=0)
 
 
-def test_glyph_run_full(gpu_ctx):
-    check(gpu_ctx, W.glyph_run(), max_lsb=0)
+@pytest.mark.parametrize("mode", MODES)
+def test_glyph_run_full(gpu_ctx, mode):
+    check(gpu_ctx, W.glyph_run(), mode, max_lsb=0)
 
 
-def test_star_blobs_small(gpu_ctx):
+@pytest.mark.parametrize("mode", MODES)
+def test_star_blobs_small(gpu_ctx, mode):
+    # blobs exceed a wave's tile grid: the batched call hands them to the plain pipeline
     b = W.star_blobs()
     order = np.argsort(np.diff(b.verb_off))
-    check(gpu_ctx, b.subset(order[:3]), max_lsb=0)
+    check(gpu_ctx, b.subset(order[:3]), mode, max_lsb=0)
 
 
-def test_adjoint_glyphs(gpu_ctx):
-    check(gpu_ctx, W.adjoint_glyphs(), max_lsb=0)
+@pytest.mark.parametrize("mode", MODES)
+def test_adjoint_glyphs(gpu_ctx, mode):
+    check(gpu_ctx, W.adjoint_glyphs(), mode, max_lsb=0)
 
 
-def test_edge_cases(gpu_ctx):
+@pytest.mark.parametrize("mode", MODES)
+def test_edge_cases(gpu_ctx, mode):
     M, L, Q, C, K, Z = W.MOVE, W.LINE, W.QUAD, W.CUBIC, W.CONIC, W.CLOSE
     paths = [
         ([], []),                                            # empty: one zero tile at (0, 0)
@@ -80,7 +88,7 @@ def test_edge_cases(gpu_ctx):
         items.append((v, np.asarray(p, np.float32).reshape(-1, 2), weights.get(i, [])))
     b = W.from_paths(items, W.IDENTITY)
     b.name = "edge_cases"
-    check(gpu_ctx, b, max_lsb=0)
+    check(gpu_ctx, b, mode, max_lsb=0)
 
 
 def test_rasterizer_api_event_order(gpu_ctx):
@@ -130,3 +138,31 @@ def test_rasterizer_api_raw_commands_and_multiple_fills(gpu_ctx):
     for a, b in zip(rec.events, ref):
         if a[0] == "tile":
             assert np.array_equal(a[3], b[3])
+
+
+def test_batched_params_and_graph_replays(gpu_ctx):
+    """Every combination of persistent waves per CU, paths per queue pop and
+    graph replay gives the oracle's result, on several consecutive replays
+    with host work and synchronisation between them."""
+    import ctypes
+
+    import torch
+
+    from ochre_amd.device import DeviceOutput, rasterize_batched, upload
+    batch = W.concat([W.glyph_run(n=1500, seed=9), W.random_paths(200, 4)])
+    batch.name = "glyphs+random"
+    ref = oracle_ref.run_batch(batch)
+    db = upload(batch)
+    lib = gpu_ctx.lib
+    try:
+        for waves, gran, graph in [(2, 1, 1), (6, 1, 1), (8, 4, 1), (4, 16, 0), (1, 2, 0)]:
+            assert lib.ochre_context_set_batched_params(gpu_ctx.h, waves, gran, graph) == 0
+            for rep in range(3):
+                out = DeviceOutput(batch.n_paths, ref.n_tiles + 16, ref.n_spans + 16)
+                rasterize_batched(gpu_ctx, db, out)
+                torch.cuda.synchronize()
+                st = compare(ref, out.to_host(), max_lsb=0)
+                print(f"[batched] waves={waves} gran={gran} graph={graph} rep={rep}: {st}", flush=True)
+                _ = np.sum(np.arange(1000))  # host work between replays
+    finally:
+        lib.ochre_context_set_batched_params(gpu_ctx.h, 6, 1, 1)
