@@ -89,7 +89,22 @@ def main():
     ap.add_argument("--mode", default="auto", choices=["auto", "plain", "batched"])
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device 5 times (scales rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
+
+    if args.calibrate_copy:
+        import torch
+        n = args.calibrate_copy << 20
+        a = torch.empty(n, dtype=torch.uint8, device="cuda")
+        b = torch.empty(n, dtype=torch.uint8, device="cuda")
+        a.fill_(1)
+        torch.cuda.synchronize()
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": n, "copies": 5}))
+        return
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build's path runs on a single MI355X "
@@ -123,6 +138,14 @@ def main():
     log(f"tiles={n_tiles} spans={n_spans} algorithmic bytes in={b_in} out={b_out}")
 
     have_batched = hasattr(ctx.lib, "ochre_rasterize_batched")
+    tuned = None
+    if have_batched and args.mode != "plain":
+        from ochre_amd.device import autotune
+        best, allm = autotune(ctx, db, out, reps=3)
+        tuned = best
+        log(f"autotune: best {best} over {len(allm)} configurations")
+        for m in sorted(allm, key=lambda m: m["ms"])[:5]:
+            log(f"  {m}")
     res = {}
     modes = ["plain"] + (["batched"] if have_batched and args.mode != "plain" else [])
     if args.mode == "batched" and not have_batched:
@@ -171,6 +194,7 @@ def main():
         "batched_paths_per_s": (batch.n_paths / res["batched"]["ms"] * 1e3) if "batched" in res else None,
         "batched_over_plain": (res["plain"]["ms"] / res["batched"]["ms"]) if "batched" in res else None,
         "hbm_frac_end_to_end": (b_in + b_out) / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "batched_params": tuned,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": res[head]["kernel"],
                      "kernel_ms": kms, "algorithmic_bytes": alg},

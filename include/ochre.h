@@ -135,6 +135,8 @@ typedef struct ochre_batch_output {
     int32_t* path_span_count; /* [n_paths] */
     int64_t n_tiles;         /* out */
     int64_t n_spans;         /* out */
+    float* tile_coverage;    /* optional, batched pipeline only: [tile_capacity][64] pre-quantisation
+                                coverage min(|accum + area|, 1) (rasterizer.rs:235 before * 256) */
 } ochre_batch_output;
 
 typedef struct ochre_context ochre_context;
@@ -162,10 +164,49 @@ int ochre_rasterize_batched(ochre_context* ctx, const ochre_path_batch* in, ochr
                             void* stream);
 
 /* Tunables of the batched pipeline (the autotuner sets them): persistent
- * waves per compute unit (1-32), paths taken per queue pop (1-1024), and
- * whether to replay the launch from a HIP graph. */
-int ochre_context_set_batched_params(ochre_context* ctx, int waves_per_cu, int paths_per_pop, int use_graph);
-int ochre_context_get_batched_params(ochre_context* ctx, int* waves_per_cu, int* paths_per_pop, int* use_graph);
+ * waves per compute unit (1-32), paths taken per queue pop (1-1024), tiles
+ * accumulated per pass in LDS (16 or 32: fewer tiles, more waves per CU),
+ * and whether to replay the launch from a HIP graph. */
+int ochre_context_set_batched_params(ochre_context* ctx, int waves_per_cu, int paths_per_pop, int tiles_per_chunk,
+                                     int use_graph);
+int ochre_context_get_batched_params(ochre_context* ctx, int* waves_per_cu, int* paths_per_pop, int* tiles_per_chunk,
+                                     int* use_graph);
+
+/* Coverage adjoint.  The loss of path p is the sum, over the pixels of the
+ * union of its tiles and its target's tiles, of (cov - target)^2, where a
+ * tile pixel's cov is min(|accum + area|, 1) and a pixel outside a render's
+ * tiles has coverage 1 inside one of its spans and 0 elsewhere.  Writes the
+ * f32 loss per path, dL/d(points) (path space, pairs, same layout as the
+ * input points) and dL/d(transform) (m00 m01 m10 m11 ox oy per path).  The
+ * target is a render in the ochre_batch_output layout (tile_coverage set).
+ * Paths must fit one wave (<= 32 x 32 tile grid, <= 256 tiles, <= 1024
+ * commands, <= 4096 points, <= 8192 lines): OCHRE_ERR_TOO_LARGE otherwise.
+ * No reference counterpart (the reference has no derivatives). */
+typedef struct ochre_coverage_target {
+    const int64_t* tile_start;
+    const int32_t* tile_count;
+    const int16_t* tile_xy;
+    const float* tile_coverage;
+    const int64_t* span_start;
+    const int32_t* span_count;
+    const int16_t* span_xy;
+    const uint16_t* span_width;
+} ochre_coverage_target;
+int ochre_coverage_adjoint(ochre_context* ctx, const ochre_path_batch* in, const ochre_coverage_target* target,
+                           float* loss, float* d_points, float* d_transforms, void* stream);
+
+/* Autotuner: times every combination of waves per CU {2,3,4,6,8,12},
+ * paths per pop {1,2,4,8} and tiles per chunk {16,32} on the caller's batch
+ * (`reps` timed calls each, median), keeps the fastest in the context and
+ * reports every measurement in `results` (up to max_results entries). */
+typedef struct ochre_batched_tuning {
+    int waves_per_cu;
+    int paths_per_pop;
+    int tiles_per_chunk;
+    float ms;
+} ochre_batched_tuning;
+int ochre_autotune_batched(ochre_context* ctx, const ochre_path_batch* in, ochre_batch_output* out, void* stream,
+                           int reps, ochre_batched_tuning* results, int max_results, int* n_results);
 
 /* Kernel-time accounting for measurement: when enabled, the pipeline
  * records HIP events around its dominant kernel; ochre_context_timing

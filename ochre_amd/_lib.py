@@ -26,7 +26,7 @@ EXPORTS = [
     "ochre_rasterizer_free", "ochre_context_create", "ochre_context_destroy",
     "ochre_rasterize_plain", "ochre_context_set_timing", "ochre_context_timing",
     "ochre_build_info", "ochre_rasterize_batched", "ochre_context_set_batched_params",
-    "ochre_context_get_batched_params",
+    "ochre_context_get_batched_params", "ochre_autotune_batched", "ochre_coverage_adjoint",
 ]
 
 
@@ -56,7 +56,18 @@ class BatchOutputC(ctypes.Structure):
                 ("span_xy", ctypes.c_void_p), ("span_width", ctypes.c_void_p), ("span_capacity", ctypes.c_int64),
                 ("path_tile_start", ctypes.c_void_p), ("path_tile_count", ctypes.c_void_p),
                 ("path_span_start", ctypes.c_void_p), ("path_span_count", ctypes.c_void_p),
-                ("n_tiles", ctypes.c_int64), ("n_spans", ctypes.c_int64)]
+                ("n_tiles", ctypes.c_int64), ("n_spans", ctypes.c_int64), ("tile_coverage", ctypes.c_void_p)]
+
+
+class CoverageTargetC(ctypes.Structure):
+    _fields_ = [("tile_start", ctypes.c_void_p), ("tile_count", ctypes.c_void_p), ("tile_xy", ctypes.c_void_p),
+                ("tile_coverage", ctypes.c_void_p), ("span_start", ctypes.c_void_p), ("span_count", ctypes.c_void_p),
+                ("span_xy", ctypes.c_void_p), ("span_width", ctypes.c_void_p)]
+
+
+class BatchedTuningC(ctypes.Structure):
+    _fields_ = [("waves_per_cu", ctypes.c_int), ("paths_per_pop", ctypes.c_int), ("tiles_per_chunk", ctypes.c_int),
+                ("ms", ctypes.c_float)]
 
 
 TILE_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int16, ctypes.c_int16, ctypes.POINTER(ctypes.c_uint8))
@@ -102,9 +113,13 @@ def _optional(lib):
     P = ctypes.c_void_p
     if hasattr(lib, "ochre_rasterize_batched"):
         lib.ochre_rasterize_batched.argtypes = [P, ctypes.POINTER(PathBatchC), ctypes.POINTER(BatchOutputC), P]
-        lib.ochre_context_set_batched_params.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int]
-        lib.ochre_context_get_batched_params.argtypes = [P, ctypes.POINTER(ctypes.c_int),
-                                                         ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+        I = ctypes.c_int
+        lib.ochre_context_set_batched_params.argtypes = [P, I, I, I, I]
+        lib.ochre_context_get_batched_params.argtypes = [P] + [ctypes.POINTER(I)] * 4
+        lib.ochre_coverage_adjoint.argtypes = [P, ctypes.POINTER(PathBatchC), ctypes.POINTER(CoverageTargetC), P, P,
+                                               P, P]
+        lib.ochre_autotune_batched.argtypes = [P, ctypes.POINTER(PathBatchC), ctypes.POINTER(BatchOutputC), P, I,
+                                               ctypes.POINTER(BatchedTuningC), I, ctypes.POINTER(I)]
 
 
 def check(status: int, what: str):

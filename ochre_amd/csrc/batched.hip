@@ -32,7 +32,6 @@ namespace ochre {
 constexpr int F_LCAP = 8192;   // lines per path
 constexpr int F_ICAP = 8192;   // increments per commit batch
 constexpr int F_GRID = 32;     // tile grid side (256 px)
-constexpr int F_TC = 32;       // tiles per accumulator chunk
 constexpr int F_TSTRIDE = 65;  // accumulator stride per tile (bank spread)
 constexpr size_t F_SCRATCH = (size_t)F_LCAP * 16 + (size_t)(F_LCAP + 1) * 4 + 12 + (size_t)F_ICAP * 16;
 constexpr uint32_t F_INVALID = 0xffffffffu;
@@ -52,6 +51,7 @@ struct FusedIn {
 struct FusedOut {
     int16_t* tile_xy;
     uint8_t* tile_alpha;
+    float* tile_cov;  // optional: pre-quantisation coverage min(|accum + area|, 1), 64 per tile
     int64_t tile_cap;
     int16_t* span_xy;
     uint16_t* span_w;
@@ -90,24 +90,115 @@ __device__ inline V2 fload(const FusedIn& in, int64_t g, const Xform& X) {
     return apply(X, mk(in.points[2 * g], in.points[2 * g + 1]));
 }
 
+template <int TC>
 struct LdsState {
     uint32_t occ[F_GRID];
-    int32_t tsign[F_GRID * F_GRID];
     uint32_t row_rank[F_GRID + 1];
-    uint32_t chunk_tile[F_TC];
-    float acc_a[F_TC * F_TSTRIDE];
-    float acc_h[F_TC * F_TSTRIDE];
-    float cin[F_TC * 8];
+    uint32_t chunk_tile[TC];
+    // tsign is dead once spans are written; the accumulators live after that
+    union {
+        int32_t tsign[F_GRID * F_GRID];
+        struct {
+            float acc_a[TC * F_TSTRIDE];
+            float acc_h[TC * F_TSTRIDE];
+            float cin[TC * 8];
+        } a;
+    } u;
 };
 
 // Rank of tile (tx, ty) (grid-local) among the path's occupied tiles in
 // row-major order.
-__device__ inline uint32_t tile_rank(const LdsState& s, int tx, int ty) {
+template <class S>
+__device__ inline uint32_t tile_rank(const S& s, int tx, int ty) {
     return s.row_rank[ty] + (uint32_t)__popc(s.occ[ty] & ((1u << tx) - 1u));
 }
 
+// Two 64-line windows of the line list held in registers (lane i holds lines
+// base + i and base + 64 + i), so refilling lanes read a line with __shfl
+// instead of a dependent global load.
+struct LineWindow {
+    float4 w0, w1;
+    uint32_t o0, o1;  // per-line increment offsets (emit walk)
+    uint32_t base;
+    __device__ void load(const float4* lines, const uint32_t* off, uint32_t L, uint32_t b, int lane) {
+        base = b;
+        const uint32_t i0 = b + lane, i1 = b + 64 + lane;
+        w0 = i0 < L ? lines[i0] : make_float4(0, 0, 0, 0);
+        w1 = i1 < L ? lines[i1] : make_float4(0, 0, 0, 0);
+        if (off) {
+            o0 = i0 < L ? off[i0] : 0u;
+            o1 = i1 < L ? off[i1] : 0u;
+        }
+    }
+    // Fetches line `my` (must be in [base, base + 128)) for every lane.
+    __device__ float4 get(uint32_t my, uint32_t* o, bool with_off) const {
+        const uint32_t src = my - base;
+        const int sl = (int)(src & 63u);
+        float4 a, b;
+        a.x = __shfl(w0.x, sl, WAVE);
+        a.y = __shfl(w0.y, sl, WAVE);
+        a.z = __shfl(w0.z, sl, WAVE);
+        a.w = __shfl(w0.w, sl, WAVE);
+        b.x = __shfl(w1.x, sl, WAVE);
+        b.y = __shfl(w1.y, sl, WAVE);
+        b.z = __shfl(w1.z, sl, WAVE);
+        b.w = __shfl(w1.w, sl, WAVE);
+        if (with_off) {
+            uint32_t x0 = __shfl(o0, sl, WAVE), x1 = __shfl(o1, sl, WAVE);
+            *o = src < 64 ? x0 : x1;
+        }
+        return src < 64 ? a : b;
+    }
+    // Slides by 64 once `next` has passed the first window.
+    __device__ void advance(const float4* lines, const uint32_t* off, uint32_t L, uint32_t next, int lane) {
+        if (next >= base + 64) {
+            w0 = w1;
+            o0 = o1;
+            base += 64;
+            const uint32_t i1 = base + 64 + lane;
+            w1 = i1 < L ? lines[i1] : make_float4(0, 0, 0, 0);
+            if (off) o1 = i1 < L ? off[i1] : 0u;
+        }
+    }
+};
+
+// Adjoint inputs/outputs (see adjoint.hip).  The target of path p is its
+// tiles [tt_start[p], +tt_count[p]) (row-major, 64 f32 coverages each) and
+// spans [ts_start[p], +ts_count[p]).
+struct AdjArgs {
+    const int64_t* tt_start;
+    const int32_t* tt_count;
+    const int16_t* t_xy;
+    const float* t_cov;
+    const int64_t* ts_start;
+    const int32_t* ts_count;
+    const int16_t* ts_xy;
+    const uint16_t* ts_w;
+    float* loss;      // [n_paths]
+    float* d_points;  // [n_points * 2]
+    float* d_xf;      // [n_paths * 6]
+};
+
+constexpr int A_TCAP = 256;   // tiles per path
+constexpr int A_VCAP = 1024;  // commands per path
+constexpr int A_PCAP = 4096;  // points per path
+constexpr int A_SCAP = 1024;  // spans per path
+constexpr size_t A_OFF_TILES = 0;                                            // float[A_TCAP][128]: g, G
+constexpr size_t A_OFF_CMD = A_OFF_TILES + (size_t)A_TCAP * 128 * 4;         // float[A_VCAP][12]
+constexpr size_t A_OFF_CMDOFF = A_OFF_CMD + (size_t)A_VCAP * 12 * 4;         // uint32[A_VCAP + 1]
+constexpr size_t A_OFF_DW = A_OFF_CMDOFF + (size_t)(A_VCAP + 4) * 4;         // float2[A_PCAP]
+constexpr size_t A_OFF_PCMD = A_OFF_DW + (size_t)A_PCAP * 8;                 // uint32[A_PCAP]
+constexpr size_t A_OFF_SPANS = A_OFF_PCMD + (size_t)A_PCAP * 4;              // uint32[A_SCAP][2]
+constexpr size_t A_SCRATCH = A_OFF_SPANS + (size_t)A_SCAP * 8;
+
+template <int TC>
+__device__ void adjoint_backward(const FusedIn& in, const AdjArgs& adj, FusedCtl* ctl, uint8_t* sc, uint8_t* asc,
+                                 LdsState<TC>& s, int64_t p, uint32_t L, uint32_t NT, uint32_t NS, int32_t tx0,
+                                 int32_t ty0, int32_t GW, int32_t GH);
+
+template <int TC, bool ADJ>
 __device__ void fused_path(const FusedIn& in, const FusedOut& out, FusedCtl* ctl, uint32_t* deferred,
-                           uint8_t* sc, LdsState& s, int64_t p) {
+                           uint8_t* sc, LdsState<TC>& s, int64_t p, const AdjArgs& adj, uint8_t* asc) {
     const int lane = threadIdx.x;
     const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     float4* lines = reinterpret_cast<float4*>(sc);
@@ -178,6 +269,12 @@ __device__ void fused_path(const FusedIn& in, const FusedOut& out, FusedCtl* ctl
             uint32_t o = L + n_incl - n;
             command_lines(c, [&](V2 a, V2 b) { lines[o++] = make_float4(a.x, a.y, b.x, b.y); });
         }
+        if (ADJ && valid && i - vb < A_VCAP) {
+            reinterpret_cast<uint32_t*>(asc + A_OFF_CMDOFF)[i - vb] = L + n_incl - n;
+            if (i == ve - 1) reinterpret_cast<uint32_t*>(asc + A_OFF_CMDOFF)[i - vb + 1] = L + n_incl;
+            for (int k = 0; k < np; k++)
+                if (g - pb + k < A_PCAP) reinterpret_cast<uint32_t*>(asc + A_OFF_PCMD)[g - pb + k] = (uint32_t)(i - vb);
+        }
         L += chunk_total;
         cpts += __shfl(np_incl, WAVE - 1, WAVE);
         ccon += __shfl(cc_incl, WAVE - 1, WAVE);
@@ -207,13 +304,18 @@ __device__ void fused_path(const FusedIn& in, const FusedOut& out, FusedCtl* ctl
     if (has_lines && !full && bminx <= bmaxx)
         grid_from_box(true, false, (int32_t)floorf(bminx), (int32_t)floorf(bminy), (int32_t)floorf(bmaxx),
                       (int32_t)floorf(bmaxy), &tx0, &ty0, &GW, &GH);
-    if (overflow || full || GW > F_GRID || GH > F_GRID) {
-        if (lane == 0) deferred[atomicAdd(&ctl->deferred, 1u)] = (uint32_t)p;
+    if (overflow || full || GW > F_GRID || GH > F_GRID ||
+        (ADJ && (ve - vb > A_VCAP || pe - pb > A_PCAP))) {
+        if (ADJ) {
+            if (lane == 0) atomicOr(&ctl->error, (uint32_t)ERR_CAPACITY);
+        } else if (lane == 0) {
+            deferred[atomicAdd(&ctl->deferred, 1u)] = (uint32_t)p;
+        }
         return;
     }
 
     // ------------------------------------------------------- count walk
-    for (int k = lane; k < F_GRID * F_GRID; k += WAVE) s.tsign[k] = 0;
+    for (int k = lane; k < F_GRID * F_GRID; k += WAVE) s.u.tsign[k] = 0;
     if (lane < F_GRID) s.occ[lane] = 0;
     __syncthreads();
     bool bbox_err = false, walk_err = false;
@@ -224,11 +326,14 @@ __device__ void fused_path(const FusedIn& in, const FusedOut& out, FusedCtl* ctl
         uint32_t line = 0, cnt = 0, steps = 0;
         int16_t ty_prev = 0, cur_tx = 0, cur_ty = 0;
         V2 point = mk(0.0f, 0.0f);
+        LineWindow win;
+        win.load(lines, nullptr, L, 0, lane);
         while (next < L || __ballot(active)) {
             const uint64_t need = __ballot(!active);
             const uint32_t my = next + (uint32_t)__popcll(need & lt);
+            float4 ln = make_float4(0, 0, 0, 0);
+            if (need && next < L) ln = win.get(my, nullptr, false);
             if (!active && my < L) {
-                float4 ln = lines[my];
                 line = my;
                 V2 a = mk(ln.x, ln.y);
                 point = mk(ln.z, ln.w);
@@ -248,6 +353,7 @@ __device__ void fused_path(const FusedIn& in, const FusedOut& out, FusedCtl* ctl
                 }
             }
             next += (uint32_t)__popcll(need);
+            win.advance(lines, nullptr, L, next, lane);
             if (active) {
                 cnt++;
                 bool term = dda_step(d);
@@ -259,7 +365,7 @@ __device__ void fused_path(const FusedIn& in, const FusedOut& out, FusedCtl* ctl
                 if (ty != ty_prev) {
                     int lx = div8(d.x) - tx0, ly = (ty_prev < ty ? ty_prev : ty) - ty0;
                     if ((unsigned)lx < (unsigned)GW && (unsigned)ly < (unsigned)GH)
-                        atomicAdd(&s.tsign[ly * F_GRID + lx], (int)(int8_t)(int16_t)(ty - ty_prev));
+                        atomicAdd(&s.u.tsign[ly * F_GRID + lx], (int)(int8_t)(int16_t)(ty - ty_prev));
                     else bbox_err = true;
                     ty_prev = ty;
                 }
@@ -299,7 +405,7 @@ __device__ void fused_path(const FusedIn& in, const FusedOut& out, FusedCtl* ctl
     const uint32_t NT = __shfl(rt_incl, WAVE - 1, WAVE);
     int32_t row_sum = 0;
     if (row_lane)
-        for (int x = 0; x < GW; x++) row_sum += s.tsign[lane * F_GRID + x];
+        for (int x = 0; x < GW; x++) row_sum += s.u.tsign[lane * F_GRID + x];
     const int32_t rs_incl = wave_incl_scan(row_sum, OpAdd());
     const int32_t w_before = rs_incl - row_sum;  // winding of all crossings in earlier rows
     // spans (rasterizer.rs:252-264): between consecutive tiles of a row
@@ -308,7 +414,7 @@ __device__ void fused_path(const FusedIn& in, const FusedOut& out, FusedCtl* ctl
         int32_t wnd = w_before, w_prev = 0, prev = -1;
         const uint32_t o = s.occ[lane];
         for (int x = 0; x < GW; x++) {
-            wnd += s.tsign[lane * F_GRID + x];
+            wnd += s.u.tsign[lane * F_GRID + x];
             if ((o >> x) & 1u) {
                 if (prev >= 0 && x > prev + 1 && w_prev != 0) row_spans++;
                 prev = x;
@@ -318,9 +424,33 @@ __device__ void fused_path(const FusedIn& in, const FusedOut& out, FusedCtl* ctl
     }
     const uint32_t rsp_incl = wave_incl_scan(row_spans, OpAdd());
     const uint32_t NS = __shfl(rsp_incl, WAVE - 1, WAVE);
+    if (ADJ) {
+        if (NT > (uint32_t)A_TCAP || NS > (uint32_t)A_SCAP) {
+            if (lane == 0) atomicOr(&ctl->error, (uint32_t)ERR_CAPACITY);
+            return;
+        }
+        if (row_lane && row_spans) {
+            uint32_t si = rsp_incl - row_spans;
+            int32_t wnd = w_before, w_prev = 0, prev = -1;
+            const uint32_t o = s.occ[lane];
+            uint32_t* sp = reinterpret_cast<uint32_t*>(asc + A_OFF_SPANS);
+            for (int x = 0; x < GW; x++) {
+                wnd += s.u.tsign[lane * F_GRID + x];
+                if ((o >> x) & 1u) {
+                    if (prev >= 0 && x > prev + 1 && w_prev != 0) {
+                        sp[2 * si] = (uint32_t)lane;                                  // grid row
+                        sp[2 * si + 1] = (uint32_t)(prev + 1) | ((uint32_t)x << 16);  // tile columns [a, b)
+                        si++;
+                    }
+                    prev = x;
+                    w_prev = wnd;
+                }
+            }
+        }
+    }
     const uint32_t NT_out = NT ? NT : 1u;
     unsigned long long tb = 0, sb = 0;
-    if (lane == 0) {
+    if (!ADJ && lane == 0) {
         tb = atomicAdd(&ctl->tiles, (unsigned long long)NT_out);
         sb = atomicAdd(&ctl->spans, (unsigned long long)NS);
         out.pts[p] = (int64_t)tb;
@@ -330,17 +460,17 @@ __device__ void fused_path(const FusedIn& in, const FusedOut& out, FusedCtl* ctl
     }
     tb = __shfl(tb, 0, WAVE);
     sb = __shfl(sb, 0, WAVE);
-    if (tb + NT_out > (unsigned long long)out.tile_cap || sb + NS > (unsigned long long)out.span_cap) {
+    if (!ADJ && (tb + NT_out > (unsigned long long)out.tile_cap || sb + NS > (unsigned long long)out.span_cap)) {
         if (lane == 0) atomicOr(&ctl->error, (uint32_t)ERR_CAPACITY);
         return;
     }
-    if (row_lane && row_spans) {
+    if (!ADJ && row_lane && row_spans) {
         uint64_t si = sb + (rsp_incl - row_spans);
         int32_t wnd = w_before, w_prev = 0, prev = -1;
         const uint32_t o = s.occ[lane];
         const int32_t ty = ty0 + lane;
         for (int x = 0; x < GW; x++) {
-            wnd += s.tsign[lane * F_GRID + x];
+            wnd += s.u.tsign[lane * F_GRID + x];
             if ((o >> x) & 1u) {
                 if (prev >= 0 && x > prev + 1 && w_prev != 0) {
                     out.span_xy[2 * si] = (int16_t)((tx0 + prev + 1) * TILE);
@@ -354,7 +484,13 @@ __device__ void fused_path(const FusedIn& in, const FusedOut& out, FusedCtl* ctl
         }
     }
     if (NT == 0) {  // no increments: one all-zero tile at (0, 0) (rasterizer.rs:193-198)
+        if (ADJ) {
+            adjoint_backward<TC>(in, adj, ctl, sc, asc, s, p, L, 0, NS, tx0, ty0, GW, GH);
+            return;
+        }
         if (lane < 4) reinterpret_cast<uint4*>(out.tile_alpha + tb * 64)[lane] = make_uint4(0, 0, 0, 0);
+        if (out.tile_cov)
+            for (int k = lane; k < 64; k += WAVE) out.tile_cov[tb * 64 + k] = 0.0f;
         if (lane == 0) {
             out.tile_xy[2 * tb] = 0;
             out.tile_xy[2 * tb + 1] = 0;
@@ -379,8 +515,8 @@ __device__ void fused_path(const FusedIn& in, const FusedOut& out, FusedCtl* ctl
     // ------------------------------------- chunks of tiles: emit, commit
     float carry = 0.0f;
     int carry_row = -1;
-    for (uint32_t r0 = 0; r0 < NT; r0 += F_TC) {
-        const uint32_t nt = NT - r0 < (uint32_t)F_TC ? NT - r0 : (uint32_t)F_TC;
+    for (uint32_t r0 = 0; r0 < NT; r0 += TC) {
+        const uint32_t nt = NT - r0 < (uint32_t)TC ? NT - r0 : (uint32_t)TC;
         if ((uint32_t)lane < nt) {
             uint32_t rank = r0 + lane;
             int ty = 0;
@@ -391,9 +527,9 @@ __device__ void fused_path(const FusedIn& in, const FusedOut& out, FusedCtl* ctl
             int tx = __ffs(o) - 1;
             s.chunk_tile[lane] = ((uint32_t)ty << 8) | (uint32_t)tx;
         }
-        for (int k = lane; k < F_TC * F_TSTRIDE; k += WAVE) {
-            s.acc_a[k] = 0.0f;
-            s.acc_h[k] = 0.0f;
+        for (int k = lane; k < TC * F_TSTRIDE; k += WAVE) {
+            s.u.a.acc_a[k] = 0.0f;
+            s.u.a.acc_h[k] = 0.0f;
         }
         __syncthreads();
         uint32_t l0 = 0;
@@ -420,21 +556,26 @@ __device__ void fused_path(const FusedIn& in, const FusedOut& out, FusedCtl* ctl
                 uint32_t pos = 0;
                 int16_t cur_tx = 0, cur_ty = 0;
                 uint32_t slot_base = F_INVALID;
+                LineWindow win;
+                win.load(lines, incoff, l1, l0, lane);
                 while (next < l1 || __ballot(active)) {
                     const uint64_t need = __ballot(!active);
                     const uint32_t my = next + (uint32_t)__popcll(need & lt);
+                    float4 ln = make_float4(0, 0, 0, 0);
+                    uint32_t lo_off = 0;
+                    if (need && next < l1) ln = win.get(my, &lo_off, true);
                     if (!active && my < l1) {
-                        float4 ln = lines[my];
                         V2 a = mk(ln.x, ln.y), b = mk(ln.z, ln.w);
                         if (!veq(a, b)) {
                             dda_init(d, a, b);
-                            pos = incoff[my] - base;
+                            pos = lo_off - base;
                             cur_tx = (int16_t)0x7fff;
                             cur_ty = (int16_t)0x7fff;
                             active = true;
                         }
                     }
                     next += (uint32_t)__popcll(need);
+                    win.advance(lines, incoff, l1, next, lane);
                     if (active) {
                         int16_t tx = div8(d.x), ty = div8(d.y);
                         if (tx != cur_tx || ty != cur_ty) {
@@ -454,9 +595,14 @@ __device__ void fused_path(const FusedIn& in, const FusedOut& out, FusedCtl* ctl
             }
             __syncthreads();
             // commit in emission order
+            const float4 inv = make_float4(__uint_as_float(F_INVALID), 0.0f, 0.0f, 0.0f);
+            float4 pf0 = (uint32_t)lane < n_inc ? incs[lane] : inv;
+            float4 pf1 = (uint32_t)(lane + WAVE) < n_inc ? incs[lane + WAVE] : inv;
             for (uint32_t cb = 0; cb < n_inc; cb += WAVE) {
-                const uint32_t j = cb + lane;
-                float4 v = j < n_inc ? incs[j] : make_float4(__uint_as_float(F_INVALID), 0.0f, 0.0f, 0.0f);
+                const float4 v = pf0;
+                pf0 = pf1;
+                const uint32_t jn = cb + 2 * WAVE + lane;
+                pf1 = jn < n_inc ? incs[jn] : inv;
                 const uint32_t key = __float_as_uint(v.x);
                 const bool valid = key != F_INVALID;
                 uint64_t match = __ballot(valid);
@@ -469,8 +615,8 @@ __device__ void fused_path(const FusedIn& in, const FusedOut& out, FusedCtl* ctl
                 const int rank = __popcll(match & lt);
                 for (int r = 0;; r++) {
                     if (valid && rank == r) {
-                        s.acc_a[key] += v.y;
-                        s.acc_h[key] += v.z;
+                        s.u.a.acc_a[key] += v.y;
+                        s.u.a.acc_h[key] += v.z;
                     }
                     if (!__ballot(valid && rank > r)) break;
                 }
@@ -486,29 +632,48 @@ __device__ void fused_path(const FusedIn& in, const FusedOut& out, FusedCtl* ctl
                     carry = 0.0f;
                     carry_row = ty;
                 }
-                s.cin[t * 8 + lane] = carry;
+                s.u.a.cin[t * 8 + lane] = carry;
 #pragma unroll
-                for (int x = 0; x < 8; x++) carry += s.acc_h[t * F_TSTRIDE + lane * 8 + x];
+                for (int x = 0; x < 8; x++) carry += s.u.a.acc_h[t * F_TSTRIDE + lane * 8 + x];
             }
         }
         __syncthreads();
+        if (ADJ) {
+            // keep v = accum + area of every pixel (rank order) for the loss
+            if ((uint32_t)lane < nt) {
+                const int t = lane;
+                float* tv = reinterpret_cast<float*>(asc + A_OFF_TILES) + (size_t)(r0 + t) * 128;
+#pragma unroll
+                for (int y = 0; y < 8; y++) {
+                    float accum = s.u.a.cin[t * 8 + y];
+#pragma unroll
+                    for (int x = 0; x < 8; x++) {
+                        const int k = t * F_TSTRIDE + y * 8 + x;
+                        tv[y * 8 + x] = accum + s.u.a.acc_a[k];
+                        accum += s.u.a.acc_h[k];
+                    }
+                }
+            }
+        }
         // quantise and store (rasterizer.rs:231-241)
-        if ((uint32_t)lane < nt) {
+        if (!ADJ && (uint32_t)lane < nt) {
             const int t = lane;
             const int ty = (int)(s.chunk_tile[t] >> 8), tx = (int)(s.chunk_tile[t] & 0xffu);
             uint32_t words[16];
 #pragma unroll
             for (int y = 0; y < 8; y++) {
-                float accum = s.cin[t * 8 + y];
+                float accum = s.u.a.cin[t * 8 + y];
                 uint32_t w0 = 0, w1 = 0;
 #pragma unroll
                 for (int x = 0; x < 8; x++) {
                     const int k = t * F_TSTRIDE + y * 8 + x;
-                    float v = fminf(fabsf(accum + s.acc_a[k]) * 256.0f, 255.0f);
+                    if (out.tile_cov)
+                        out.tile_cov[(tb + r0 + (uint32_t)t) * 64 + y * 8 + x] = fminf(fabsf(accum + s.u.a.acc_a[k]), 1.0f);
+                    float v = fminf(fabsf(accum + s.u.a.acc_a[k]) * 256.0f, 255.0f);
                     uint32_t b = (uint32_t)v;
                     if (x < 4) w0 |= b << (8 * x);
                     else w1 |= b << (8 * (x - 4));
-                    accum += s.acc_h[k];
+                    accum += s.u.a.acc_h[k];
                 }
                 words[2 * y] = w0;
                 words[2 * y + 1] = w1;
@@ -524,11 +689,13 @@ __device__ void fused_path(const FusedIn& in, const FusedOut& out, FusedCtl* ctl
         }
         __syncthreads();
     }
+    if (ADJ) adjoint_backward<TC>(in, adj, ctl, sc, asc, s, p, L, NT, NS, tx0, ty0, GW, GH);
 }
 
+template <int TC>
 __global__ __launch_bounds__(WAVE) void k_fused(FusedIn in, FusedOut out, FusedCtl* ctl, uint32_t* deferred,
                                                 uint8_t* scratch, uint32_t gran) {
-    __shared__ LdsState s;
+    __shared__ LdsState<TC> s;
     uint8_t* sc = scratch + (size_t)blockIdx.x * F_SCRATCH;
     for (;;) {
         uint32_t base = 0;
@@ -536,9 +703,11 @@ __global__ __launch_bounds__(WAVE) void k_fused(FusedIn in, FusedOut out, FusedC
         base = __shfl(base, 0, WAVE);
         if (base >= (uint64_t)in.n_paths) break;
         const int64_t end = std::min<int64_t>((int64_t)base + gran, in.n_paths);
-        for (int64_t p = base; p < end; p++) fused_path(in, out, ctl, deferred, sc, s, p);
+        for (int64_t p = base; p < end; p++) fused_path<TC, false>(in, out, ctl, deferred, sc, s, p, AdjArgs{}, nullptr);
     }
 }
+
+#include "adjoint.inc"
 
 // ---------------------------------------------- deferred paths -> plain
 
@@ -571,7 +740,7 @@ __global__ void k_scatter_ranges(const uint32_t* ids, uint32_t n, const int64_t*
 }
 
 struct BatchedState {
-    DBuf scratch, ctl, deferred;
+    DBuf scratch, ascratch, ctl, deferred;
     DBuf g_verbs, g_points, g_weights, g_voff, g_poff, g_woff, g_xf, g_ts, g_tc, g_ss, g_sc;
     int n_waves = 0;
     uint32_t gran = 1;
@@ -586,7 +755,7 @@ struct BatchedState {
 };
 
 int run_batched(Context* ctx, const SegInput& in, OutputDesc& out, hipStream_t s, int waves_per_cu, int gran,
-                bool use_graph) {
+                int tiles_per_chunk, bool use_graph) {
     if (!ctx->batched) ctx->batched = new BatchedState();
     BatchedState* B = reinterpret_cast<BatchedState*>(ctx->batched);
     const int64_t NP = in.n_paths;
@@ -615,6 +784,7 @@ int run_batched(Context* ctx, const SegInput& in, OutputDesc& out, hipStream_t s
     FusedOut fo;
     fo.tile_xy = out.tile_xy;
     fo.tile_alpha = out.tile_alpha;
+    fo.tile_cov = out.tile_cov;
     fo.tile_cap = out.tile_capacity;
     fo.span_xy = out.span_xy;
     fo.span_w = out.span_w;
@@ -627,10 +797,12 @@ int run_batched(Context* ctx, const SegInput& in, OutputDesc& out, hipStream_t s
     uint32_t* deferred = B->deferred.as<uint32_t>();
     uint8_t* scratch = B->scratch.as<uint8_t>();
     const uint32_t g = (uint32_t)std::max(1, gran);
+    const bool tc16 = tiles_per_chunk <= 16;
 
     auto enqueue = [&](hipStream_t st) {
         hipLaunchKernelGGL(k_fused_reset, dim3(1), dim3(64), 0, st, ctl);
-        hipLaunchKernelGGL(k_fused, dim3(n_waves), dim3(WAVE), 0, st, fi, fo, ctl, deferred, scratch, g);
+        if (tc16) hipLaunchKernelGGL(k_fused<16>, dim3(n_waves), dim3(WAVE), 0, st, fi, fo, ctl, deferred, scratch, g);
+        else hipLaunchKernelGGL(k_fused<32>, dim3(n_waves), dim3(WAVE), 0, st, fi, fo, ctl, deferred, scratch, g);
     };
     const bool timing = ctx->timing;
     if (timing) {
@@ -638,7 +810,8 @@ int run_batched(Context* ctx, const SegInput& in, OutputDesc& out, hipStream_t s
         if (!ctx->ev[1]) HIPCHK_RET(hipEventCreate(&ctx->ev[1]));
     }
     if (use_graph) {
-        std::vector<uint8_t> key(sizeof(fi) + sizeof(fo) + 16);
+        std::vector<uint8_t> key(sizeof(fi) + sizeof(fo) + 17);
+        key.back() = tc16 ? 1 : 0;
         memcpy(key.data(), &fi, sizeof(fi));
         memcpy(key.data() + sizeof(fi), &fo, sizeof(fo));
         memcpy(key.data() + sizeof(fi) + sizeof(fo), &n_waves, 4);
@@ -665,7 +838,8 @@ int run_batched(Context* ctx, const SegInput& in, OutputDesc& out, hipStream_t s
     } else {
         hipLaunchKernelGGL(k_fused_reset, dim3(1), dim3(64), 0, s, ctl);
         if (timing) HIPCHK_RET(hipEventRecord(ctx->ev[0], s));
-        hipLaunchKernelGGL(k_fused, dim3(n_waves), dim3(WAVE), 0, s, fi, fo, ctl, deferred, scratch, g);
+        if (tc16) hipLaunchKernelGGL(k_fused<16>, dim3(n_waves), dim3(WAVE), 0, s, fi, fo, ctl, deferred, scratch, g);
+        else hipLaunchKernelGGL(k_fused<32>, dim3(n_waves), dim3(WAVE), 0, s, fi, fo, ctl, deferred, scratch, g);
         if (timing) HIPCHK_RET(hipEventRecord(ctx->ev[1], s));
     }
     HIPCHK_RET(hipGetLastError());
@@ -764,6 +938,76 @@ int run_batched(Context* ctx, const SegInput& in, OutputDesc& out, hipStream_t s
     out.n_spans = Sf + Sp;
     if ((hc->error & ERR_CAPACITY) || out.n_tiles > out.tile_capacity || out.n_spans > out.span_capacity) return -4;
     return st;
+}
+
+int run_adjoint(Context* ctx, const SegInput& in, const AdjTarget& tgt, float* loss, float* d_points, float* d_xf,
+                hipStream_t s, int waves_per_cu, int gran, int tiles_per_chunk) {
+    if (!ctx->batched) ctx->batched = new BatchedState();
+    BatchedState* B = reinterpret_cast<BatchedState*>(ctx->batched);
+    const int64_t NP = in.n_paths;
+    if (NP <= 0) return 0;
+    int dev = 0, cus = 0;
+    HIPCHK_RET(hipGetDevice(&dev));
+    HIPCHK_RET(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const int n_waves = std::max(1, cus * std::max(1, waves_per_cu));
+    HIPCHK_RET(B->scratch.ensure((size_t)n_waves * F_SCRATCH));
+    HIPCHK_RET(B->ascratch.ensure((size_t)n_waves * A_SCRATCH));
+    HIPCHK_RET(B->ctl.ensure(sizeof(FusedCtl)));
+    if (!ctx->host_pinned) HIPCHK_RET(hipHostMalloc(&ctx->host_pinned, 4096));
+    FusedIn fi;
+    fi.n_paths = NP;
+    fi.verbs = in.verbs;
+    fi.verb_off = in.seg_verb_off;
+    fi.points = in.points;
+    fi.point_off = in.seg_point_off;
+    fi.weights = in.weights;
+    fi.weight_off = in.seg_weight_off;
+    fi.xf = in.seg_xf;
+    fi.n_points = in.n_points;
+    fi.n_weights = in.n_weights;
+    AdjArgs a;
+    a.tt_start = tgt.tile_start;
+    a.tt_count = tgt.tile_count;
+    a.t_xy = tgt.tile_xy;
+    a.t_cov = tgt.tile_cov;
+    a.ts_start = tgt.span_start;
+    a.ts_count = tgt.span_count;
+    a.ts_xy = tgt.span_xy;
+    a.ts_w = tgt.span_w;
+    a.loss = loss;
+    a.d_points = d_points;
+    a.d_xf = d_xf;
+    FusedCtl* ctl = B->ctl.as<FusedCtl>();
+    const uint32_t g = (uint32_t)std::max(1, gran);
+    hipLaunchKernelGGL(k_fused_reset, dim3(1), dim3(64), 0, s, ctl);
+    if (ctx->timing) {
+        if (!ctx->ev[0]) HIPCHK_RET(hipEventCreate(&ctx->ev[0]));
+        if (!ctx->ev[1]) HIPCHK_RET(hipEventCreate(&ctx->ev[1]));
+        HIPCHK_RET(hipEventRecord(ctx->ev[0], s));
+    }
+    if (tiles_per_chunk <= 16)
+        hipLaunchKernelGGL(k_adjoint<16>, dim3(n_waves), dim3(WAVE), 0, s, fi, a, ctl, B->scratch.as<uint8_t>(),
+                           B->ascratch.as<uint8_t>(), g);
+    else
+        hipLaunchKernelGGL(k_adjoint<32>, dim3(n_waves), dim3(WAVE), 0, s, fi, a, ctl, B->scratch.as<uint8_t>(),
+                           B->ascratch.as<uint8_t>(), g);
+    HIPCHK_RET(hipGetLastError());
+    if (ctx->timing) HIPCHK_RET(hipEventRecord(ctx->ev[1], s));
+    FusedCtl* hc = reinterpret_cast<FusedCtl*>(reinterpret_cast<uint8_t*>(ctx->host_pinned) + 1024);
+    HIPCHK_RET(hipMemcpyAsync(hc, ctl, sizeof(FusedCtl), hipMemcpyDeviceToHost, s));
+    HIPCHK_RET(hipStreamSynchronize(s));
+    if (ctx->timing) {
+        float ms = 0;
+        HIPCHK_RET(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+        auto& t = ctx->timings["k_adjoint"];
+        t.first += ms;
+        t.second += 1;
+    }
+    if (hc->error & ERR_INPUT) return -1;
+    if (hc->error & ERR_FLATTEN) return -5;
+    if (hc->error & ERR_BBOX) return -3;
+    if (hc->error & ERR_CAPACITY) return -7;
+    return 0;
 }
 
 }  // namespace ochre

@@ -8,6 +8,8 @@
 // batch entry points return OCHRE_ERR_NO_DEVICE.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -157,6 +159,7 @@ static OutputDesc to_output(const ochre_batch_output* out) {
     OutputDesc od;
     od.tile_xy = out->tile_xy;
     od.tile_alpha = out->tile_alpha;
+    od.tile_cov = out->tile_coverage;
     od.tile_capacity = out->tile_capacity;
     od.span_xy = out->span_xy;
     od.span_w = out->span_width;
@@ -177,28 +180,98 @@ int ochre_rasterize_batched(ochre_context* ctx, const ochre_path_batch* in, ochr
     if (hipSetDevice(ctx->c.device) != hipSuccess) return OCHRE_ERR_DEVICE;
     OutputDesc od = to_output(out);
     st = run_batched(&ctx->c, si, od, reinterpret_cast<hipStream_t>(stream), ctx->c.batched_waves_per_cu,
-                     ctx->c.batched_gran, ctx->c.batched_graph);
+                     ctx->c.batched_gran, ctx->c.batched_tc, ctx->c.batched_graph);
     out->n_tiles = od.n_tiles;
     out->n_spans = od.n_spans;
     return st;
 }
 
-int ochre_context_set_batched_params(ochre_context* ctx, int waves_per_cu, int paths_per_pop, int use_graph) {
-    if (!ctx || waves_per_cu < 1 || waves_per_cu > 32 || paths_per_pop < 1 || paths_per_pop > 1024)
+int ochre_context_set_batched_params(ochre_context* ctx, int waves_per_cu, int paths_per_pop, int tiles_per_chunk,
+                                     int use_graph) {
+    if (!ctx || waves_per_cu < 1 || waves_per_cu > 32 || paths_per_pop < 1 || paths_per_pop > 1024 ||
+        (tiles_per_chunk != 16 && tiles_per_chunk != 32))
         return OCHRE_ERR_INVALID_ARG;
     std::lock_guard<std::mutex> lk(ctx->mu);
     ctx->c.batched_waves_per_cu = waves_per_cu;
     ctx->c.batched_gran = paths_per_pop;
+    ctx->c.batched_tc = tiles_per_chunk;
     ctx->c.batched_graph = use_graph != 0;
     return OCHRE_OK;
 }
 
-int ochre_context_get_batched_params(ochre_context* ctx, int* waves_per_cu, int* paths_per_pop, int* use_graph) {
+int ochre_context_get_batched_params(ochre_context* ctx, int* waves_per_cu, int* paths_per_pop, int* tiles_per_chunk,
+                                     int* use_graph) {
     if (!ctx) return OCHRE_ERR_INVALID_ARG;
     std::lock_guard<std::mutex> lk(ctx->mu);
     if (waves_per_cu) *waves_per_cu = ctx->c.batched_waves_per_cu;
     if (paths_per_pop) *paths_per_pop = ctx->c.batched_gran;
+    if (tiles_per_chunk) *tiles_per_chunk = ctx->c.batched_tc;
     if (use_graph) *use_graph = ctx->c.batched_graph ? 1 : 0;
+    return OCHRE_OK;
+}
+
+int ochre_coverage_adjoint(ochre_context* ctx, const ochre_path_batch* in, const ochre_coverage_target* target,
+                           float* loss, float* d_points, float* d_transforms, void* stream) {
+    if (!ctx || !in || !target || !loss || !d_points || !d_transforms) return OCHRE_ERR_INVALID_ARG;
+    if (in->n_paths > 0 && (!target->tile_start || !target->tile_count || !target->span_start || !target->span_count))
+        return OCHRE_ERR_INVALID_ARG;
+    SegInput si;
+    int st = to_seg_input(in, &si);
+    if (st) return st;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (hipSetDevice(ctx->c.device) != hipSuccess) return OCHRE_ERR_DEVICE;
+    AdjTarget t;
+    t.tile_start = target->tile_start;
+    t.tile_count = target->tile_count;
+    t.tile_xy = target->tile_xy;
+    t.tile_cov = target->tile_coverage;
+    t.span_start = target->span_start;
+    t.span_count = target->span_count;
+    t.span_xy = target->span_xy;
+    t.span_w = target->span_width;
+    return run_adjoint(&ctx->c, si, t, loss, d_points, d_transforms, reinterpret_cast<hipStream_t>(stream),
+                       ctx->c.batched_waves_per_cu, ctx->c.batched_gran, ctx->c.batched_tc);
+}
+
+int ochre_autotune_batched(ochre_context* ctx, const ochre_path_batch* in, ochre_batch_output* out, void* stream,
+                           int reps, ochre_batched_tuning* results, int max_results, int* n_results) {
+    if (!ctx || !in || !out || reps < 1) return OCHRE_ERR_INVALID_ARG;
+    SegInput si;
+    int st = to_seg_input(in, &si);
+    if (st) return st;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (hipSetDevice(ctx->c.device) != hipSuccess) return OCHRE_ERR_DEVICE;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    static const int waves[] = {2, 3, 4, 6, 8, 12};
+    static const int grans[] = {1, 2, 4, 8};
+    static const int tcs[] = {16, 32};
+    ochre_batched_tuning best{ctx->c.batched_waves_per_cu, ctx->c.batched_gran, ctx->c.batched_tc, 1e30f};
+    int k = 0;
+    for (int w : waves)
+        for (int g : grans)
+            for (int tc : tcs) {
+                OutputDesc od = to_output(out);
+                // warm-up (also captures the graph for this configuration)
+                st = run_batched(&ctx->c, si, od, s, w, g, tc, true);
+                if (st) return st;
+                std::vector<double> t;
+                for (int r = 0; r < reps; r++) {
+                    auto t0 = std::chrono::steady_clock::now();
+                    st = run_batched(&ctx->c, si, od, s, w, g, tc, true);
+                    if (st) return st;
+                    t.push_back(std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+                }
+                std::sort(t.begin(), t.end());
+                const float ms = (float)t[t.size() / 2];
+                if (results && k < max_results) results[k] = ochre_batched_tuning{w, g, tc, ms};
+                k++;
+                if (ms < best.ms) best = ochre_batched_tuning{w, g, tc, ms};
+            }
+    if (n_results) *n_results = k;
+    ctx->c.batched_waves_per_cu = best.waves_per_cu;
+    ctx->c.batched_gran = best.paths_per_pop;
+    ctx->c.batched_tc = best.tiles_per_chunk;
+    ctx->c.batched_graph = true;
     return OCHRE_OK;
 }
 

@@ -36,6 +36,7 @@ struct SegInput {
 struct OutputDesc {
     int16_t* tile_xy = nullptr;
     uint8_t* tile_alpha = nullptr;
+    float* tile_cov = nullptr;  // optional f32 pre-quantisation coverage (batched pipeline)
     int64_t tile_capacity = 0;
     int16_t* span_xy = nullptr;
     uint16_t* span_w = nullptr;
@@ -108,7 +109,8 @@ struct Context {
     // batched-path state lives in batched.hip
     void* batched = nullptr;
     int batched_waves_per_cu = 6;
-    int batched_gran = 1;
+    int batched_gran = 2;
+    int batched_tc = 32;
     bool batched_graph = true;
     ~Context();
 };
@@ -119,7 +121,22 @@ int run_plain(Context* ctx, const SegInput& in, OutputDesc& out, hipStream_t str
 // gran the paths taken per queue pop; use_graph replays (reset, kernel) from
 // a captured HIP graph.
 int run_batched(Context* ctx, const SegInput& in, OutputDesc& out, hipStream_t stream, int waves_per_cu, int gran,
-                bool use_graph);
+                int tiles_per_chunk, bool use_graph);
+
+// Target render of the coverage adjoint (device pointers, per path ranges).
+struct AdjTarget {
+    const int64_t* tile_start;
+    const int32_t* tile_count;
+    const int16_t* tile_xy;
+    const float* tile_cov;
+    const int64_t* span_start;
+    const int32_t* span_count;
+    const int16_t* span_xy;
+    const uint16_t* span_w;
+};
+// Loss and gradients of the f32 coverage (one wave per path).
+int run_adjoint(Context* ctx, const SegInput& in, const AdjTarget& tgt, float* loss, float* d_points, float* d_xf,
+                hipStream_t stream, int waves_per_cu, int gran, int tiles_per_chunk);
 
 #define HIPCHK_RET(x)                                      \
     do {                                                   \

@@ -99,7 +99,7 @@ def upload(batch, device="cuda") -> DeviceBatch:
 class DeviceOutput:
     """Caller-owned output arrays (see ochre_batch_output)."""
 
-    def __init__(self, n_paths: int, tile_capacity: int, span_capacity: int, device="cuda"):
+    def __init__(self, n_paths: int, tile_capacity: int, span_capacity: int, device="cuda", coverage=False):
         torch = _torch()
         self.n_paths = n_paths
         self.tile_capacity = max(int(tile_capacity), 1)
@@ -113,6 +113,8 @@ class DeviceOutput:
         self.path_tile_count = torch.empty((n,), dtype=torch.int32, device=device)
         self.path_span_start = torch.empty((n,), dtype=torch.int64, device=device)
         self.path_span_count = torch.empty((n,), dtype=torch.int32, device=device)
+        self.tile_cov = (torch.empty((self.tile_capacity, 64), dtype=torch.float32, device=device)
+                         if coverage else None)
         self.n_tiles = 0
         self.n_spans = 0
 
@@ -120,7 +122,16 @@ class DeviceOutput:
         return _lib.BatchOutputC(self.tile_xy.data_ptr(), self.tile_alpha.data_ptr(), self.tile_capacity,
                                  self.span_xy.data_ptr(), self.span_w.data_ptr(), self.span_capacity,
                                  self.path_tile_start.data_ptr(), self.path_tile_count.data_ptr(),
-                                 self.path_span_start.data_ptr(), self.path_span_count.data_ptr(), 0, 0)
+                                 self.path_span_start.data_ptr(), self.path_span_count.data_ptr(), 0, 0,
+                                 self.tile_cov.data_ptr() if self.tile_cov is not None else None)
+
+    def target(self) -> _lib.CoverageTargetC:
+        """This render (with coverage) as an adjoint target."""
+        assert self.tile_cov is not None
+        return _lib.CoverageTargetC(self.path_tile_start.data_ptr(), self.path_tile_count.data_ptr(),
+                                    self.tile_xy.data_ptr(), self.tile_cov.data_ptr(),
+                                    self.path_span_start.data_ptr(), self.path_span_count.data_ptr(),
+                                    self.span_xy.data_ptr(), self.span_w.data_ptr())
 
     def to_host(self):
         """Per-path host arrays in path order: (tile_xy, tile_alpha, span_xy,
@@ -194,3 +205,65 @@ def run_to_host(ctx, batch, mode="plain", tile_capacity=None, span_capacity=None
             tc, sc = out.n_tiles, out.n_spans
     torch.cuda.synchronize()
     return out.to_host()
+
+
+def batched_params(ctx: Context):
+    w, g, tc, gr = (ctypes.c_int() for _ in range(4))
+    _lib.check(ctx.lib.ochre_context_get_batched_params(ctx.h, ctypes.byref(w), ctypes.byref(g), ctypes.byref(tc),
+                                                        ctypes.byref(gr)), "get_batched_params")
+    return {"waves_per_cu": w.value, "paths_per_pop": g.value, "tiles_per_chunk": tc.value, "graph": gr.value}
+
+
+def set_batched_params(ctx: Context, waves_per_cu, paths_per_pop, tiles_per_chunk, graph=True):
+    _lib.check(ctx.lib.ochre_context_set_batched_params(ctx.h, waves_per_cu, paths_per_pop, tiles_per_chunk,
+                                                        1 if graph else 0), "set_batched_params")
+
+
+def autotune(ctx: Context, db: DeviceBatch, out: DeviceOutput, reps=3, cache_key=None, cache_path=None, stream=None):
+    """Runs the native autotuner on this batch (or reuses a cached choice for
+    `cache_key`) and leaves the best parameters set in the context.  Returns
+    (best, all_measurements)."""
+    import json
+    import os
+    path = cache_path or os.environ.get("OCHRE_AUTOTUNE_CACHE") or os.path.expanduser("~/.cache/ochre_amd/autotune.json")
+    if cache_key is not None and os.path.exists(path):
+        try:
+            cached = json.load(open(path)).get(cache_key)
+        except Exception:
+            cached = None
+        if cached:
+            set_batched_params(ctx, cached["waves_per_cu"], cached["paths_per_pop"], cached["tiles_per_chunk"])
+            return cached, []
+    res = (_lib.BatchedTuningC * 64)()
+    n = ctypes.c_int()
+    cb, co = db.c(), out.c()
+    _lib.check(ctx.lib.ochre_autotune_batched(ctx.h, ctypes.byref(cb), ctypes.byref(co), _stream_ptr(stream), reps,
+                                              res, 64, ctypes.byref(n)), "ochre_autotune_batched")
+    allm = [{"waves_per_cu": r.waves_per_cu, "paths_per_pop": r.paths_per_pop, "tiles_per_chunk": r.tiles_per_chunk,
+             "ms": r.ms} for r in res[:n.value]]
+    best = dict(batched_params(ctx))
+    best["ms"] = min(m["ms"] for m in allm) if allm else None
+    if cache_key is not None:
+        try:
+            os.makedirs(os.path.dirname(path), exist_ok=True)
+            data = json.load(open(path)) if os.path.exists(path) else {}
+            data[cache_key] = best
+            json.dump(data, open(path, "w"), indent=1)
+        except Exception:
+            pass
+    return best, allm
+
+
+def coverage_adjoint(ctx: Context, db: DeviceBatch, target: DeviceOutput, stream=None):
+    """Loss per path and gradients w.r.t. the path-space points and the
+    transforms (torch tensors on the device)."""
+    torch = _torch()
+    loss = torch.empty((max(db.n_paths, 1),), dtype=torch.float32, device="cuda")
+    dp = torch.empty((max(db.n_points, 1), 2), dtype=torch.float32, device="cuda")
+    dx = torch.empty((max(db.n_paths, 1), 6), dtype=torch.float32, device="cuda")
+    cb = db.c()
+    ct = target.target()
+    _lib.check(ctx.lib.ochre_coverage_adjoint(ctx.h, ctypes.byref(cb), ctypes.byref(ct), loss.data_ptr(),
+                                              dp.data_ptr(), dx.data_ptr(), _stream_ptr(stream)),
+               "ochre_coverage_adjoint")
+    return loss[:db.n_paths], dp[:db.n_points], dx[:db.n_paths]

@@ -218,3 +218,54 @@ def stroke_path(verbs, points, width):
         return st, _take_path(cp)
     finally:
         lib.orc_path_free(ctypes.byref(cp))
+
+
+# ------------------------------------------------------ f64 coverage adjoint
+
+def _bind_adjoint(lib):
+    if getattr(lib, "_adj_bound", False):
+        return
+    P = ctypes.c_void_p
+    S = ctypes.c_size_t
+    lib.orc64_path_loss.restype = ctypes.c_int
+    lib.orc64_path_loss.argtypes = [P, S, P, S, P, S, P, P, P, S, P, P, S, P, P, P, P, P, S, P]
+    lib._adj_bound = True
+
+
+class Target:
+    """A render used as the loss target: tiles (xy, f32 coverage) + spans."""
+
+    def __init__(self, txy, cov, sxy, sw):
+        self.txy = np.ascontiguousarray(txy, np.int16).reshape(-1, 2)
+        self.cov = np.ascontiguousarray(cov, np.float32).reshape(-1, 64)
+        self.sxy = np.ascontiguousarray(sxy, np.int16).reshape(-1, 2)
+        self.sw = np.ascontiguousarray(sw, np.uint16).reshape(-1)
+
+
+EMPTY_TARGET = None
+
+
+def path_loss64(verbs, points, weights, xf, target=None, grad=False, want_cov=False, cov_cap=4096):
+    """f64 forward coverage, loss against `target` and (grad=True) the
+    gradient w.r.t. the path-space points and the transform.  Returns
+    (status, loss, dpts, dxf, (tile_xy, cov))."""
+    lib = load()
+    _bind_adjoint(lib)
+    v = np.ascontiguousarray(verbs, np.uint8)
+    p = np.ascontiguousarray(np.asarray(points, np.float64).reshape(-1, 2))
+    w = np.ascontiguousarray(weights, np.float64).reshape(-1)
+    x = np.ascontiguousarray(xf, np.float64).reshape(6)
+    t = target or Target(np.zeros((0, 2)), np.zeros((0, 64)), np.zeros((0, 2)), np.zeros(0))
+    loss = ctypes.c_double()
+    dp = np.zeros_like(p) if grad else None
+    dx = np.zeros(6) if grad else None
+    oxy = np.zeros((cov_cap, 2), np.int16) if want_cov else None
+    oc = np.zeros((cov_cap, 64)) if want_cov else None
+    nt = ctypes.c_size_t()
+    st = lib.orc64_path_loss(_ptr(v), v.size, _ptr(p), p.shape[0], _ptr(w), w.size, _ptr(x),
+                             _ptr(t.txy), _ptr(t.cov), t.txy.shape[0], _ptr(t.sxy), _ptr(t.sw), t.sw.size,
+                             ctypes.byref(loss), _ptr(dp) if grad else None, _ptr(dx) if grad else None,
+                             _ptr(oxy) if want_cov else None, _ptr(oc) if want_cov else None, cov_cap,
+                             ctypes.byref(nt))
+    cov = (oxy[:nt.value], oc[:nt.value]) if want_cov else None
+    return st, loss.value, dp, dx, cov

@@ -155,14 +155,14 @@ def test_batched_params_and_graph_replays(gpu_ctx):
     db = upload(batch)
     lib = gpu_ctx.lib
     try:
-        for waves, gran, graph in [(2, 1, 1), (6, 1, 1), (8, 4, 1), (4, 16, 0), (1, 2, 0)]:
-            assert lib.ochre_context_set_batched_params(gpu_ctx.h, waves, gran, graph) == 0
+        for waves, gran, tc, graph in [(2, 1, 32, 1), (6, 1, 16, 1), (8, 4, 32, 1), (4, 16, 16, 0), (1, 2, 32, 0)]:
+            assert lib.ochre_context_set_batched_params(gpu_ctx.h, waves, gran, tc, graph) == 0
             for rep in range(3):
                 out = DeviceOutput(batch.n_paths, ref.n_tiles + 16, ref.n_spans + 16)
                 rasterize_batched(gpu_ctx, db, out)
                 torch.cuda.synchronize()
                 st = compare(ref, out.to_host(), max_lsb=0)
-                print(f"[batched] waves={waves} gran={gran} graph={graph} rep={rep}: {st}", flush=True)
+                print(f"[batched] waves={waves} gran={gran} tc={tc} graph={graph} rep={rep}: {st}", flush=True)
                 _ = np.sum(np.arange(1000))  # host work between replays
     finally:
-        lib.ochre_context_set_batched_params(gpu_ctx.h, 6, 1, 1)
+        lib.ochre_context_set_batched_params(gpu_ctx.h, 6, 2, 32, 1)

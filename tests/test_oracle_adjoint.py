@@ -1,0 +1,78 @@
+"""Pins the f64 coverage adjoint of the oracle:
+
+* its forward (f64) quantised like the reference agrees with the f32 oracle
+  within 1 LSB and emits the same tile set;
+* its analytic gradient matches central finite differences (h = 1e-3 px in
+  world units) of its own forward, on control points and transform entries,
+  away from kinks (one-sided differences are compared to detect them).
+"""
+import numpy as np
+import pytest
+
+import oracle_ref as O
+from ochre_amd import workloads as W
+
+
+def target_from(batch_path, perturb, rng):
+    v, p, w, xf = batch_path
+    pp = p.astype(np.float64) + rng.normal(0, perturb, p.shape) / max(abs(float(xf[0])), 1e-3)
+    st, _, _, _, (txy, cov) = O.path_loss64(v, pp, w, xf, want_cov=True)
+    assert st == 0
+    return O.Target(txy, cov.astype(np.float32), np.zeros((0, 2)), np.zeros(0))
+
+
+def test_f64_forward_matches_f32_oracle_within_1_lsb():
+    b = W.adjoint_glyphs(n=40)
+    ref = O.run_batch(b)
+    for i in range(b.n_paths):
+        v, p, w, xf = b.path(i)
+        st, _, _, _, (txy, cov) = O.path_loss64(v, p.astype(np.float64), w, xf.astype(np.float64), want_cov=True)
+        t0, t1 = ref.path_tile_off[i], ref.path_tile_off[i + 1]
+        assert np.array_equal(txy, ref.tile_xy[t0:t1])
+        a = np.minimum(np.floor(cov * 256.0), 255).astype(np.int16)
+        assert np.abs(a - ref.tile_alpha[t0:t1].astype(np.int16)).max() <= 1
+
+
+def fd(fun, x, i, h):
+    xp = x.copy()
+    xp[i] += h
+    xm = x.copy()
+    xm[i] -= h
+    lp, lm, l0 = fun(xp), fun(xm), fun(x)
+    c = (lp - lm) / (2 * h)
+    right, left = (lp - l0) / h, (l0 - lm) / h
+    return c, right, left
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_adjoint_matches_finite_differences(seed):
+    rng = np.random.default_rng(700 + seed)
+    b = W.adjoint_glyphs(n=8, seed=50 + seed)
+    checked = 0
+    for i in range(b.n_paths):
+        v, p, w, xf = b.path(i)
+        p64 = p.astype(np.float64)
+        xf64 = xf.astype(np.float64)
+        tgt = target_from((v, p, w, xf64), 0.5, rng)
+        st, L, dp, dx, _ = O.path_loss64(v, p64, w, xf64, tgt, grad=True)
+        assert st == 0
+        s = float(xf64[0])
+        # control points: h = 1e-3 px in world space
+        for _ in range(4):
+            k = int(rng.integers(0, p64.size))
+            fun = lambda z: O.path_loss64(v, z.reshape(-1, 2), w, xf64, tgt)[1]
+            c, r, l = fd(fun, p64.reshape(-1), k, 1e-3 / s)
+            if abs(r - l) > 1e-3 * max(abs(c), 1.0):
+                continue  # a kink inside [x - h, x + h]
+            g = dp.reshape(-1)[k]
+            assert abs(g - c) <= 1e-3 * max(abs(c), 1.0) + 1e-6, (i, k, g, c)
+            checked += 1
+        for k in range(6):
+            hh = 1e-3 / (s if k < 4 else 1.0) / 32.0 if k < 4 else 1e-3
+            fun = lambda z: O.path_loss64(v, p64, w, z, tgt)[1]
+            c, r, l = fd(fun, xf64, k, hh)
+            if abs(r - l) > 1e-3 * max(abs(c), 1.0):
+                continue
+            assert abs(dx[k] - c) <= 1e-3 * max(abs(c), 1.0) + 1e-6, (i, "xf", k, dx[k], c)
+            checked += 1
+    assert checked >= 20
