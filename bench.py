@@ -89,6 +89,9 @@ def main():
     ap.add_argument("--mode", default="auto", choices=["auto", "plain", "batched"])
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--params", default=None, metavar="W,G,TC",
+                    help="batched pipeline parameters (waves/CU, paths per pop, tiles per chunk) instead of autotuning")
+    ap.add_argument("--no-graph", action="store_true", help="launch the batched kernel directly, not from a graph")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 5 times (scales rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -140,12 +143,22 @@ def main():
     have_batched = hasattr(ctx.lib, "ochre_rasterize_batched")
     tuned = None
     if have_batched and args.mode != "plain":
-        from ochre_amd.device import autotune
-        best, allm = autotune(ctx, db, out, reps=3)
-        tuned = best
-        log(f"autotune: best {best} over {len(allm)} configurations")
-        for m in sorted(allm, key=lambda m: m["ms"])[:5]:
-            log(f"  {m}")
+        from ochre_amd.device import autotune, batched_params, set_batched_params
+        if args.params:
+            w, g, tc = (int(x) for x in args.params.split(","))
+            set_batched_params(ctx, w, g, tc, graph=not args.no_graph)
+            tuned = batched_params(ctx)
+            log(f"batched parameters set: {tuned}")
+        else:
+            best, allm = autotune(ctx, db, out, reps=3)
+            if args.no_graph:
+                set_batched_params(ctx, best["waves_per_cu"], best["paths_per_pop"], best["tiles_per_chunk"],
+                                   graph=False)
+            tuned = batched_params(ctx)
+            tuned["ms"] = best["ms"]
+            log(f"autotune: best {best} over {len(allm)} configurations")
+            for m in sorted(allm, key=lambda m: m["ms"])[:5]:
+                log(f"  {m}")
     res = {}
     modes = ["plain"] + (["batched"] if have_batched and args.mode != "plain" else [])
     if args.mode == "batched" and not have_batched:

@@ -10,6 +10,7 @@
  *   ochre_rasterizer_line_to    Rasterizer::line_to    src/rasterizer.rs:72-140
  *   ochre_rasterizer_command    Rasterizer::command    src/rasterizer.rs:145-157
  *   ochre_rasterizer_fill       Rasterizer::fill       src/rasterizer.rs:161-165
+ *   ochre_rasterizer_stroke     Rasterizer::stroke     src/rasterizer.rs:169-171
  *   ochre_rasterizer_finish     Rasterizer::finish     src/rasterizer.rs:180-268
  *   ochre_tile_builder          trait TileBuilder      src/rasterizer.rs:12-22
  *   ochre_transform             Transform / Mat2x2     src/geom.rs:131-133,203-207
@@ -92,6 +93,12 @@ int ochre_rasterizer_command(ochre_rasterizer* r, uint8_t verb, const float* poi
 int ochre_rasterizer_fill(ochre_rasterizer* r, const uint8_t* verbs, size_t n_verbs, const float* points,
                           size_t n_points, const float* weights, size_t n_weights,
                           const ochre_transform* transform);
+/* Rasterizer::stroke: the path is flattened and stroked (butt caps, miter
+ * joins; path.rs:114-274) in path space with the given width, and the outline
+ * is filled with the transform.  Both expansions run on the GPU at finish. */
+int ochre_rasterizer_stroke(ochre_rasterizer* r, const uint8_t* verbs, size_t n_verbs, const float* points,
+                            size_t n_points, const float* weights, size_t n_weights, float width,
+                            const ochre_transform* transform);
 /* Rasterizer::finish: rasterises on the GPU, then calls the builder.
  * Consumes the rasterizer (the handle is freed, whatever the status). */
 int ochre_rasterizer_finish(ochre_rasterizer* r, const ochre_tile_builder* builder);
@@ -162,6 +169,17 @@ int ochre_rasterize_plain(ochre_context* ctx, const ochre_path_batch* in, ochre_
  * row-major order, but paths are laid out in the order they finish. */
 int ochre_rasterize_batched(ochre_context* ctx, const ochre_path_batch* in, ochre_batch_output* out,
                             void* stream);
+
+/* Rasterizer::stroke for every path of a batch (widths: [n_paths], device
+ * memory): flattening and stroke expansion on the device (one lane per path),
+ * then the plain or batched fill of the outlines with the paths' transforms.
+ * Returns OCHRE_ERR_INVALID_ARG for verbs that do not match their points or
+ * weights and OCHRE_ERR_FLATTEN_LIMIT for a curve the reference would not
+ * finish flattening. */
+int ochre_stroke_plain(ochre_context* ctx, const ochre_path_batch* in, const float* widths, ochre_batch_output* out,
+                       void* stream);
+int ochre_stroke_batched(ochre_context* ctx, const ochre_path_batch* in, const float* widths,
+                         ochre_batch_output* out, void* stream);
 
 /* Tunables of the batched pipeline (the autotuner sets them): persistent
  * waves per compute unit (1-32), paths taken per queue pop (1-1024), tiles

@@ -192,7 +192,8 @@ class Rasterizer:
         pts = np.asarray([(p.x, p.y) for p in cmd.points], np.float32).reshape(-1)
         _lib.check(self._lib.ochre_rasterizer_command(self._h, cmd.verb, _ptr(pts), cmd.weight), "command")
 
-    def fill_arrays(self, verbs, points, weights=(), transform=None):
+    @staticmethod
+    def _arrays(verbs, points, weights, transform):
         v = np.ascontiguousarray(verbs, np.uint8)
         p = np.ascontiguousarray(np.asarray(points, np.float32).reshape(-1, 2))
         w = np.ascontiguousarray(weights, np.float32).reshape(-1)
@@ -206,12 +207,26 @@ class Rasterizer:
             for i in range(4):
                 tc.m[i] = float(a[i])
             tc.offset[0], tc.offset[1] = float(a[4]), float(a[5])
+        return v, p, w, tc
+
+    def fill_arrays(self, verbs, points, weights=(), transform=None):
+        v, p, w, tc = self._arrays(verbs, points, weights, transform)
         _lib.check(self._lib.ochre_rasterizer_fill(self._h, _ptr(v), v.size, _ptr(p), p.shape[0], _ptr(w), w.size,
                                                    ctypes.byref(tc)), "fill")
 
     def fill(self, path, transform: Transform):
         verbs, points, weights = encode_path(path)
         self.fill_arrays(verbs, points, weights, transform)
+
+    def stroke_arrays(self, verbs, points, width, weights=(), transform=None):
+        v, p, w, tc = self._arrays(verbs, points, weights, transform)
+        _lib.check(self._lib.ochre_rasterizer_stroke(self._h, _ptr(v), v.size, _ptr(p), p.shape[0], _ptr(w), w.size,
+                                                     float(width), ctypes.byref(tc)), "stroke")
+
+    def stroke(self, path, width: float, transform: Transform):
+        """Rasterizer::stroke (rasterizer.rs:169-171): butt caps, miter joins."""
+        verbs, points, weights = encode_path(path)
+        self.stroke_arrays(verbs, points, width, weights, transform)
 
     def finish(self, builder: TileBuilder):
         """Rasterises on the GPU and replays the results into `builder`.

@@ -27,6 +27,7 @@ EXPORTS = [
     "ochre_rasterize_plain", "ochre_context_set_timing", "ochre_context_timing",
     "ochre_build_info", "ochre_rasterize_batched", "ochre_context_set_batched_params",
     "ochre_context_get_batched_params", "ochre_autotune_batched", "ochre_coverage_adjoint",
+    "ochre_rasterizer_stroke", "ochre_stroke_plain", "ochre_stroke_batched",
 ]
 
 
@@ -120,6 +121,11 @@ def _optional(lib):
                                                P, P]
         lib.ochre_autotune_batched.argtypes = [P, ctypes.POINTER(PathBatchC), ctypes.POINTER(BatchOutputC), P, I,
                                                ctypes.POINTER(BatchedTuningC), I, ctypes.POINTER(I)]
+    if hasattr(lib, "ochre_rasterizer_stroke"):
+        lib.ochre_rasterizer_stroke.argtypes = [P, P, ctypes.c_size_t, P, ctypes.c_size_t, P, ctypes.c_size_t,
+                                                ctypes.c_float, P]
+        for name in ("ochre_stroke_plain", "ochre_stroke_batched"):
+            getattr(lib, name).argtypes = [P, ctypes.POINTER(PathBatchC), P, ctypes.POINTER(BatchOutputC), P]
 
 
 def check(status: int, what: str):

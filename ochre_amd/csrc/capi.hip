@@ -186,6 +186,31 @@ int ochre_rasterize_batched(ochre_context* ctx, const ochre_path_batch* in, ochr
     return st;
 }
 
+static int stroke_batch(ochre_context* ctx, const ochre_path_batch* in, const float* widths,
+                        ochre_batch_output* out, void* stream, bool batched) {
+    if (!ctx || !in || !out || (in->n_paths > 0 && !widths)) return OCHRE_ERR_INVALID_ARG;
+    SegInput si;
+    int st = to_seg_input(in, &si);
+    if (st) return st;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (hipSetDevice(ctx->c.device) != hipSuccess) return OCHRE_ERR_DEVICE;
+    OutputDesc od = to_output(out);
+    st = run_stroke(&ctx->c, si, widths, od, reinterpret_cast<hipStream_t>(stream), batched);
+    out->n_tiles = od.n_tiles;
+    out->n_spans = od.n_spans;
+    return st;
+}
+
+int ochre_stroke_plain(ochre_context* ctx, const ochre_path_batch* in, const float* widths, ochre_batch_output* out,
+                       void* stream) {
+    return stroke_batch(ctx, in, widths, out, stream, false);
+}
+
+int ochre_stroke_batched(ochre_context* ctx, const ochre_path_batch* in, const float* widths,
+                         ochre_batch_output* out, void* stream) {
+    return stroke_batch(ctx, in, widths, out, stream, true);
+}
+
 int ochre_context_set_batched_params(ochre_context* ctx, int waves_per_cu, int paths_per_pop, int tiles_per_chunk,
                                      int use_graph) {
     if (!ctx || waves_per_cu < 1 || waves_per_cu > 32 || paths_per_pop < 1 || paths_per_pop > 1024 ||
@@ -286,6 +311,12 @@ struct ochre_rasterizer {
     std::vector<int64_t> seg_verb_off{0}, seg_point_off{0}, seg_weight_off{0};
     std::vector<Xform> seg_xf;
     std::vector<uint8_t> seg_raw;
+    // stroke(): the path-space input of stroke k; segment i with
+    // seg_stroke[i] == k is empty until finish() puts k's outline there.
+    std::vector<int32_t> seg_stroke;
+    std::vector<uint8_t> s_verbs;
+    std::vector<float> s_points, s_weights, s_width;
+    std::vector<int64_t> s_voff{0}, s_poff{0}, s_woff{0};
     ochre_context* ctx = nullptr;
 
     void open_segment(bool raw, const Xform& xf) {
@@ -294,6 +325,7 @@ struct ochre_rasterizer {
         seg_weight_off.push_back((int64_t)weights.size());
         seg_xf.push_back(xf);
         seg_raw.push_back(raw ? 1 : 0);
+        seg_stroke.push_back(-1);
     }
     // Segment ends are the next segment's starts; the trailing entries are
     // kept equal to the current sizes.
@@ -308,6 +340,7 @@ struct ochre_rasterizer {
             // seg_*_off holds n_segs + 1 entries: the last one is the end.
             seg_xf.push_back(id);
             seg_raw.push_back(1);
+            seg_stroke.push_back(-1);
             seg_verb_off.push_back((int64_t)verbs.size());
             seg_point_off.push_back((int64_t)points.size() / 2);
             seg_weight_off.push_back((int64_t)weights.size());
@@ -325,6 +358,7 @@ int verb_npts(uint8_t v) { return ochre::verb_points(v); }
 
 struct DevPath {
     DBuf verbs, points, weights, seg_voff, seg_poff, seg_woff, seg_xf, seg_raw, seg_path, path_seg;
+    DBuf s_verbs, s_points, s_weights, s_voff, s_poff, s_woff, s_width;
     DBuf tile_xy, tile_alpha, span_xy, span_w, pts, ptc, pss, psc;
 };
 
@@ -336,6 +370,73 @@ int upload(DBuf& b, const std::vector<T>& v, hipStream_t s) {
         e = hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s);
         if (e != hipSuccess) return map_hip_error(e);
     }
+    return 0;
+}
+
+// Runs the recorded strokes through the device expansion (flatten + stroke)
+// and splices each outline into its placeholder segment.  The outlines come
+// back to the host because the segment layout is rebuilt there; all of their
+// arithmetic happens on the device.
+int expand_strokes(ochre_context* ctx, ochre_rasterizer* r, DevPath* dp, hipStream_t s) {
+    const int64_t K = (int64_t)r->s_width.size();
+    std::vector<Xform> xf((size_t)K, Xform{1, 0, 0, 1, 0, 0});
+    int st;
+    if ((st = upload(dp->s_verbs, r->s_verbs, s)) || (st = upload(dp->s_points, r->s_points, s)) ||
+        (st = upload(dp->s_weights, r->s_weights, s)) || (st = upload(dp->s_voff, r->s_voff, s)) ||
+        (st = upload(dp->s_poff, r->s_poff, s)) || (st = upload(dp->s_woff, r->s_woff, s)) ||
+        (st = upload(dp->s_width, r->s_width, s)) || (st = upload(dp->seg_xf, xf, s)))
+        return st;
+    SegInput si;
+    si.n_segs = K;
+    si.n_paths = K;
+    si.verbs = dp->s_verbs.as<uint8_t>();
+    si.points = dp->s_points.as<float>();
+    si.weights = dp->s_weights.as<float>();
+    si.seg_verb_off = dp->s_voff.as<int64_t>();
+    si.seg_point_off = dp->s_poff.as<int64_t>();
+    si.seg_weight_off = dp->s_woff.as<int64_t>();
+    si.seg_xf = dp->seg_xf.as<Xform>();
+    si.n_verbs = (int64_t)r->s_verbs.size();
+    si.n_points = (int64_t)r->s_points.size() / 2;
+    si.n_weights = (int64_t)r->s_weights.size();
+    StrokeOutline o;
+    if ((st = stroke_expand(&ctx->c, si, dp->s_width.as<float>(), o, s))) return st;
+    std::vector<int64_t> ov((size_t)K + 1), op((size_t)K + 1);
+    std::vector<uint8_t> overbs((size_t)o.in.n_verbs);
+    std::vector<float> opts((size_t)o.in.n_points * 2);
+    hipError_t e = hipMemcpyAsync(ov.data(), o.in.seg_verb_off, ov.size() * 8, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(op.data(), o.in.seg_point_off, op.size() * 8, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess && !overbs.empty())
+        e = hipMemcpyAsync(overbs.data(), o.in.verbs, overbs.size(), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess && !opts.empty())
+        e = hipMemcpyAsync(opts.data(), o.in.points, opts.size() * 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return map_hip_error(e);
+    // rebuild the segment layout with every outline in place
+    std::vector<uint8_t> verbs;
+    std::vector<float> points;
+    std::vector<int64_t> voff{0}, poff{0}, woff{0};
+    const size_t nseg = r->seg_xf.size();
+    for (size_t i = 0; i < nseg; i++) {
+        const int32_t k = r->seg_stroke[i];
+        if (k < 0) {
+            verbs.insert(verbs.end(), r->verbs.begin() + r->seg_verb_off[i], r->verbs.begin() + r->seg_verb_off[i + 1]);
+            points.insert(points.end(), r->points.begin() + 2 * r->seg_point_off[i],
+                          r->points.begin() + 2 * r->seg_point_off[i + 1]);
+        } else {
+            verbs.insert(verbs.end(), overbs.begin() + ov[k], overbs.begin() + ov[k + 1]);
+            points.insert(points.end(), opts.begin() + 2 * op[k], opts.begin() + 2 * op[k + 1]);
+        }
+        voff.push_back((int64_t)verbs.size());
+        poff.push_back((int64_t)points.size() / 2);
+        woff.push_back(r->seg_weight_off[i + 1]);  // outlines add no weights
+    }
+    r->verbs.swap(verbs);
+    r->points.swap(points);
+    r->seg_verb_off.swap(voff);
+    r->seg_point_off.swap(poff);
+    r->seg_weight_off.swap(woff);
+    std::fill(r->seg_stroke.begin(), r->seg_stroke.end(), -1);
     return 0;
 }
 
@@ -376,10 +477,10 @@ int ochre_rasterizer_command(ochre_rasterizer* r, uint8_t verb, const float* poi
     return OCHRE_OK;
 }
 
-int ochre_rasterizer_fill(ochre_rasterizer* r, const uint8_t* verbs, size_t n_verbs, const float* points,
-                          size_t n_points, const float* weights, size_t n_weights,
-                          const ochre_transform* transform) {
-    if (!r || !transform) return OCHRE_ERR_INVALID_ARG;
+}  // extern "C"
+
+static int check_path(const uint8_t* verbs, size_t n_verbs, const float* points, size_t n_points,
+                      const float* weights, size_t n_weights) {
     if ((n_verbs && !verbs) || (n_points && !points) || (n_weights && !weights)) return OCHRE_ERR_INVALID_ARG;
     size_t np = 0, nw = 0;
     for (size_t i = 0; i < n_verbs; i++) {
@@ -388,12 +489,47 @@ int ochre_rasterizer_fill(ochre_rasterizer* r, const uint8_t* verbs, size_t n_ve
         np += (size_t)k;
         nw += verbs[i] == OCHRE_VERB_CONIC;
     }
-    if (np != n_points || nw != n_weights) return OCHRE_ERR_INVALID_ARG;
+    return np == n_points && nw == n_weights ? OCHRE_OK : OCHRE_ERR_INVALID_ARG;
+}
+
+extern "C" {
+
+int ochre_rasterizer_stroke(ochre_rasterizer* r, const uint8_t* verbs, size_t n_verbs, const float* points,
+                            size_t n_points, const float* weights, size_t n_weights, float width,
+                            const ochre_transform* transform) {
+    if (!r || !transform) return OCHRE_ERR_INVALID_ARG;
+    int st = check_path(verbs, n_verbs, points, n_points, weights, n_weights);
+    if (st) return st;
+    Xform xf;
+    memcpy(&xf, transform, sizeof(xf));
+    r->s_verbs.insert(r->s_verbs.end(), verbs, verbs + n_verbs);
+    r->s_points.insert(r->s_points.end(), points, points + 2 * n_points);
+    r->s_weights.insert(r->s_weights.end(), weights, weights + n_weights);
+    r->s_voff.push_back((int64_t)r->s_verbs.size());
+    r->s_poff.push_back((int64_t)r->s_points.size() / 2);
+    r->s_woff.push_back((int64_t)r->s_weights.size());
+    r->s_width.push_back(width);
+    r->seg_xf.push_back(xf);
+    r->seg_raw.push_back(0);
+    r->seg_stroke.push_back((int32_t)r->s_width.size() - 1);
+    r->seg_verb_off.push_back((int64_t)r->verbs.size());
+    r->seg_point_off.push_back((int64_t)r->points.size() / 2);
+    r->seg_weight_off.push_back((int64_t)r->weights.size());
+    return OCHRE_OK;
+}
+
+int ochre_rasterizer_fill(ochre_rasterizer* r, const uint8_t* verbs, size_t n_verbs, const float* points,
+                          size_t n_points, const float* weights, size_t n_weights,
+                          const ochre_transform* transform) {
+    if (!r || !transform) return OCHRE_ERR_INVALID_ARG;
+    int st = check_path(verbs, n_verbs, points, n_points, weights, n_weights);
+    if (st) return st;
     Xform xf;
     memcpy(&xf, transform, sizeof(xf));
     // open a transformed segment
     r->seg_xf.push_back(xf);
     r->seg_raw.push_back(0);
+    r->seg_stroke.push_back(-1);
     r->seg_verb_off.push_back((int64_t)r->verbs.size());
     r->seg_point_off.push_back((int64_t)r->points.size() / 2);
     r->seg_weight_off.push_back((int64_t)r->weights.size());
@@ -421,11 +557,13 @@ int ochre_rasterizer_finish(ochre_rasterizer* r, const ochre_tile_builder* build
             break;
         }
         hipStream_t s = nullptr;
+        if (!r->s_width.empty() && (st = expand_strokes(ctx, r, dp, s))) break;
         // a path with no segments is one empty segment
         if (r->seg_xf.empty()) {
             Xform id{1, 0, 0, 1, 0, 0};
             r->seg_xf.push_back(id);
             r->seg_raw.push_back(1);
+            r->seg_stroke.push_back(-1);
             r->seg_verb_off.push_back(0);
             r->seg_point_off.push_back(0);
             r->seg_weight_off.push_back(0);

@@ -87,6 +87,13 @@ struct DBuf {
     T* as() const { return reinterpret_cast<T*>(p); }
 };
 
+// Device buffers of the stroke expansion (flattened polylines, outlines).
+struct StrokeBuffers {
+    DBuf cnt, fv, fp, ov, op, ow, err, fverbs, fpoints, overbs, opoints;
+    void* host = nullptr;  // pinned: sizes and error bits read back
+    ~StrokeBuffers();
+};
+
 struct Context {
     int device = 0;
     hipStream_t own_stream = nullptr;
@@ -112,6 +119,7 @@ struct Context {
     int batched_gran = 2;
     int batched_tc = 32;
     bool batched_graph = true;
+    StrokeBuffers stroke;
     ~Context();
 };
 
@@ -137,6 +145,18 @@ struct AdjTarget {
 // Loss and gradients of the f32 coverage (one wave per path).
 int run_adjoint(Context* ctx, const SegInput& in, const AdjTarget& tgt, float* loss, float* d_points, float* d_xf,
                 hipStream_t stream, int waves_per_cu, int gran, int tiles_per_chunk);
+
+// Outline of a stroked batch: a fill batch (one segment per path, the input's
+// transforms) whose buffers belong to the context until its next stroke call.
+struct StrokeOutline {
+    SegInput in;
+};
+// Rasterizer::stroke for every path of a batch: path-space flattening and
+// stroke expansion on the device (widths: [n_paths], device memory).
+int stroke_expand(Context* ctx, const SegInput& in, const float* widths, StrokeOutline& out, hipStream_t stream);
+// stroke_expand, then the plain or batched fill of the outlines.
+int run_stroke(Context* ctx, const SegInput& in, const float* widths, OutputDesc& out, hipStream_t stream,
+               bool batched);
 
 #define HIPCHK_RET(x)                                      \
     do {                                                   \

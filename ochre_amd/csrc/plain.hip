@@ -48,6 +48,10 @@ DBuf::~DBuf() {
     if (p) (void)hipFree(p);
 }
 
+StrokeBuffers::~StrokeBuffers() {
+    if (host) (void)hipHostFree(host);
+}
+
 Context::~Context() {
     if (host_pinned) (void)hipHostFree(host_pinned);
     for (auto& e : ev)

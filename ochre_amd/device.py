@@ -186,14 +186,33 @@ def rasterize_batched(ctx: Context, db: DeviceBatch, out: DeviceOutput, stream=N
     return co.n_tiles, co.n_spans
 
 
+def stroke(ctx: Context, db: DeviceBatch, widths, out: DeviceOutput, batched=True, stream=None):
+    """Rasterizer::stroke for every path (widths: a device float32 tensor of
+    n_paths): device flattening and stroke expansion, then the batched (or
+    plain) fill.  Returns (n_tiles, n_spans)."""
+    cb = db.c()
+    co = out.c()
+    fn = ctx.lib.ochre_stroke_batched if batched else ctx.lib.ochre_stroke_plain
+    wp = widths.data_ptr() if widths.numel() else None
+    st = fn(ctx.h, ctypes.byref(cb), wp, ctypes.byref(co), _stream_ptr(stream))
+    out.n_tiles, out.n_spans = co.n_tiles, co.n_spans
+    _lib.check(st, "ochre_stroke_batched" if batched else "ochre_stroke_plain")
+    return co.n_tiles, co.n_spans
+
+
 def run_to_host(ctx, batch, mode="plain", tile_capacity=None, span_capacity=None):
     """Convenience: upload, rasterise (growing the outputs once if needed)
-    and return the per-path host arrays."""
+    and return the per-path host arrays.  Modes: "plain", "batched",
+    "stroke_plain", "stroke_batched" (strokes use batch.widths)."""
     torch = _torch()
     db = upload(batch)
     tc = tile_capacity or max(1024, batch.n_paths * 64)
     sc = span_capacity or max(1024, batch.n_paths * 8)
-    fn = rasterize_plain if mode == "plain" else rasterize_batched
+    if mode.startswith("stroke"):
+        wd = torch.from_numpy(np.ascontiguousarray(batch.widths, np.float32)).to("cuda")
+        fn = lambda c, d, o: stroke(c, d, wd, o, batched=mode == "stroke_batched")  # noqa: E731
+    else:
+        fn = rasterize_plain if mode == "plain" else rasterize_batched
     for _ in range(2):
         out = DeviceOutput(batch.n_paths, tc, sc)
         try:

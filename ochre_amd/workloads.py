@@ -185,6 +185,35 @@ def glyph_run(n=10_000, seed=1, canvas=4096.0, smin=12.0, smax=48.0) -> PathBatc
     return PathBatch(verbs, verb_off, points, point_off, w, wo, xf, name=f"glyph_run(n={n})")
 
 
+def stroked_glyphs(n=10_000, seed=5, canvas=4096.0, smin=12.0, smax=48.0, wmin=0.75, wmax=3.0) -> PathBatch:
+    """The glyph geometry in pixel units (as SVG strokes are authored: the
+    path-space flattening tolerance is 0.1 px), stroked with widths of
+    0.75-3 px and placed by a translation."""
+    rng = np.random.default_rng(SEED_BASE + seed)
+    verbs, verb_off, points, point_off = _glyph_geometry(rng, n)
+    s = rng.uniform(smin, smax, n).astype(np.float32)
+    counts = np.diff(point_off)
+    points = (points * np.repeat(s, counts)[:, None]).astype(np.float32)
+    off = (rng.random((n, 2)) * (canvas - s[:, None])).astype(np.float32)
+    xf = np.zeros((n, 6), np.float32)
+    xf[:, 0] = 1.0
+    xf[:, 3] = 1.0
+    xf[:, 4:] = off
+    w, wo = _empty_weights(n)
+    b = PathBatch(verbs, verb_off, points, point_off, w, wo, xf, name=f"stroked_glyphs(n={n})")
+    b.widths = rng.uniform(wmin, wmax, n).astype(np.float32)
+    return b
+
+
+def with_widths(batch: PathBatch, lo: float, hi: float, seed: int) -> PathBatch:
+    """The batch with stroke widths drawn from [lo, hi) (path units)."""
+    rng = np.random.default_rng(SEED_BASE + 2000 + seed)
+    b = dataclasses.replace(batch)
+    b.widths = rng.uniform(lo, hi, batch.n_paths).astype(np.float32)
+    b.name = f"{batch.name}+widths[{lo},{hi})"
+    return b
+
+
 def _blob_points(rng, N, R, cx, cy):
     M = 3 * N
     theta = 2.0 * math.pi * np.arange(M) / M
@@ -341,4 +370,5 @@ WORKLOADS = {
     "star_blobs": star_blobs,
     "scenes": scenes,
     "adjoint_glyphs": adjoint_glyphs,
+    "stroked_glyphs": stroked_glyphs,
 }
