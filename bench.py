@@ -50,9 +50,10 @@ def cpu_baseline(batch, seconds):
     oracle_ref.load()
     reps, t0 = 0, time.perf_counter()
     times = []
+    ref = None
     while True:
         a = time.perf_counter()
-        oracle_ref.run_batch(batch)
+        ref = oracle_ref.run_batch(batch)
         times.append(time.perf_counter() - a)
         reps += 1
         if time.perf_counter() - t0 >= seconds and reps >= 2:
@@ -60,7 +61,20 @@ def cpu_baseline(batch, seconds):
     times.sort()
     med = times[len(times) // 2]
     return {"value": batch.n_paths / med, "unit": "paths/s", "cores": 1, "kind": "port",
-            "sample": f"{batch.name}: all {batch.n_paths} paths, median of {reps} passes (single thread)"}
+            "sample": f"{batch.name}: all {batch.n_paths} paths, median of {reps} passes (single thread)"}, ref
+
+
+def traffic_bytes(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload` from the committed
+    rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated), or
+    None when no summary covers them."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload.split('(')[0]}_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for ln in open(path):
+        if ln.startswith("traffic_bytes_per_launch:"):
+            return float(ln.split(":", 1)[1])
+    return None
 
 
 def time_steps(fn, steps, warmup):
@@ -91,6 +105,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--params", default=None, metavar="W,G,TC",
                     help="batched pipeline parameters (waves/CU, paths per pop, tiles per chunk) instead of autotuning")
+    ap.add_argument("--dense", default=None, metavar="G,W",
+                    help="dense kernel launch shape (workgroups per CU, waves per workgroup) instead of autotuning")
+    ap.add_argument("--no-dense", action="store_true", help="the ordered batched kernel alone (bit-exact sums)")
     ap.add_argument("--no-graph", action="store_true", help="launch the batched kernel directly, not from a graph")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 5 times (scales rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -117,7 +134,8 @@ def main():
     import torch
 
     from ochre_amd import workloads as W
-    from ochre_amd.device import Context, DeviceOutput, rasterize_batched, rasterize_plain, upload
+    from ochre_amd.device import (Context, DeviceOutput, batched_params, rasterize_batched, rasterize_plain,
+                                  set_batched_params, set_dense_params, upload)
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     batch = W.WORKLOADS[args.workload]()
@@ -143,9 +161,11 @@ def main():
     have_batched = hasattr(ctx.lib, "ochre_rasterize_batched")
     tuned = None
     if have_batched and args.mode != "plain":
-        from ochre_amd.device import autotune, batched_params, set_batched_params
-        if args.params:
-            w, g, tc = (int(x) for x in args.params.split(","))
+        from ochre_amd.device import autotune
+        dg, dw = (int(x) for x in args.dense.split(",")) if args.dense else (4, 4)
+        set_dense_params(ctx, not args.no_dense, dg, dw)
+        if args.params or (args.dense and not args.no_dense):
+            w, g, tc = (int(x) for x in (args.params or "8,1,32").split(","))
             set_batched_params(ctx, w, g, tc, graph=not args.no_graph)
             tuned = batched_params(ctx)
             log(f"batched parameters set: {tuned}")
@@ -163,23 +183,47 @@ def main():
     modes = ["plain"] + (["batched"] if have_batched and args.mode != "plain" else [])
     if args.mode == "batched" and not have_batched:
         raise SystemExit("batched pipeline not built")
+    outputs = {}
     for mode in modes:
-        fn = (lambda: rasterize_plain(ctx, db, out)) if mode == "plain" else (lambda: rasterize_batched(ctx, db, out))
-        ctx.set_timing(True)
-        ev_ms, wall_ms = time_steps(fn, args.steps, args.warmup)
-        kname = "k_tiles" if mode == "plain" else "k_fused"
-        kms, kn = ctx.timing(kname)
-        ctx.set_timing(False)
-        res[mode] = {"ms": ev_ms, "wall_ms": wall_ms, "kernel": kname,
-                     "kernel_ms": kms / max(kn, 1), "kernel_launches": kn}
+        if mode == "plain":
+            fn = lambda: rasterize_plain(ctx, db, out)  # noqa: E731
+            ctx.set_timing(True)
+            ev_ms, wall_ms = time_steps(fn, args.steps, args.warmup)
+            kms, kn = ctx.timing("k_tiles")
+            ctx.set_timing(False)
+            kernels = {"k_tiles": kms / max(kn, 1)}
+            kname = "k_tiles"
+        else:
+            fn = lambda: rasterize_batched(ctx, db, out)  # noqa: E731
+            # the production launch (graph replay unless --no-graph)
+            ev_ms, wall_ms = time_steps(fn, args.steps, args.warmup)
+            # kernel durations: the same launches made directly, with HIP
+            # events around each kernel on the stream they run on
+            bp = batched_params(ctx)
+            set_batched_params(ctx, bp["waves_per_cu"], bp["paths_per_pop"], bp["tiles_per_chunk"], graph=False)
+            ctx.set_timing(True)
+            time_steps(fn, args.steps, 1)
+            kernels = {}
+            for k in ("k_dense", "k_fused"):
+                t, n = ctx.timing(k)
+                if n:
+                    kernels[k] = t / n
+            ctx.set_timing(False)
+            set_batched_params(ctx, bp["waves_per_cu"], bp["paths_per_pop"], bp["tiles_per_chunk"],
+                               graph=bool(bp["graph"]))
+            kname = "k_dense" if bp.get("dense") else "k_fused"
+        torch.cuda.synchronize()
+        outputs[mode] = out.to_host()
+        res[mode] = {"ms": ev_ms, "wall_ms": wall_ms, "kernel": kname, "kernel_ms": kernels.get(kname, 0.0),
+                     "kernels_ms": kernels}
         log(f"{mode}: {ev_ms:.3f} ms/step (wall {wall_ms:.3f}), {batch.n_paths / ev_ms * 1e3:,.0f} paths/s, "
-            f"{kname} {kms / max(kn, 1):.3f} ms x {kn}")
+            f"kernels (ms/launch) {kernels}")
 
     head = "batched" if "batched" in res else "plain"
     ms = res[head]["ms"]
     paths_s = batch.n_paths / ms * 1e3
     kms = res[head]["kernel_ms"]
-    # roofline of the dominant kernel: the fused kernel moves the whole
+    # roofline of the dominant kernel: the batched kernels move the whole
     # interface (input paths + output tiles/spans); the plain pipeline's tile
     # kernel writes the output and reads nothing of the interface.
     alg = (b_in + b_out) if head == "batched" else b_out
@@ -208,15 +252,19 @@ def main():
         "batched_over_plain": (res["plain"]["ms"] / res["batched"]["ms"]) if "batched" in res else None,
         "hbm_frac_end_to_end": (b_in + b_out) / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
         "batched_params": tuned,
+        "kernels_ms": {m: r["kernels_ms"] for m, r in res.items()},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": res[head]["kernel"],
-                     "kernel_ms": kms, "algorithmic_bytes": alg},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_bytes(res[head]["kernel"], batch.name),
+                     "kernel": res[head]["kernel"], "kernel_ms": kms, "algorithmic_bytes": alg},
     }
     if not args.no_cpu:
-        cb = cpu_baseline(batch, args.cpu_seconds)
+        cb, ref = cpu_baseline(batch, args.cpu_seconds)
         cb["cpu"] = platform.processor() or platform.machine()
         line["cpu_baseline"] = cb
         line["vs_cpu_1core"] = paths_s / cb["value"]
+        # the oracle's result from the baseline leg checks every timed output
+        from parity import compare
+        line["parity"] = {m: compare(ref, o, max_lsb=1) for m, o in outputs.items()}
     else:
         line["cpu_baseline"] = None
     print(json.dumps(line), flush=True)

@@ -181,6 +181,28 @@ int ochre_stroke_plain(ochre_context* ctx, const ochre_path_batch* in, const flo
 int ochre_stroke_batched(ochre_context* ctx, const ochre_path_batch* in, const float* widths,
                          ochre_batch_output* out, void* stream);
 
+/* Device-side tile consumer: the demo's atlas Builder (examples/svg.rs:15-88)
+ * applied to a batch's device output, as if one Rasterizer per path finished
+ * into one Builder in path order (svg.rs:140-156).  Writes the R8 atlas
+ * (atlas_size^2 bytes, atlas_size a multiple of 8: slot 0 solid, tile g of
+ * the batch in slot g + 1, row-major slots of 8x8, unused slots cleared),
+ * 4 vertices and 6 indices per TileBuilder call, in call order (tiles
+ * textured from their slot, spans at uv 0), colour colors[p] (4 bytes per
+ * path, device memory).  `in` is an ochre_batch_output filled by any of the
+ * rasterize/stroke entry points for these n_paths.  OCHRE_ERR_CAPACITY when
+ * the tiles do not fit the atlas or the vertex/index arrays are too small
+ * (*n_vertices / *n_indices report the sizes needed).  Synchronises the
+ * stream once to size the output. */
+typedef struct ochre_atlas_vertex {
+    int16_t pos[2];  /* svg.rs:15-20 Vertex */
+    uint16_t uv[2];
+    uint8_t col[4];
+} ochre_atlas_vertex;
+int ochre_build_atlas(ochre_context* ctx, const ochre_batch_output* in, int64_t n_paths, const uint8_t* colors,
+                      uint8_t* atlas, int atlas_size, ochre_atlas_vertex* vertices, int64_t vertex_capacity,
+                      uint32_t* indices, int64_t index_capacity, int64_t* n_vertices, int64_t* n_indices,
+                      void* stream);
+
 /* Tunables of the batched pipeline (the autotuner sets them): persistent
  * waves per compute unit (1-32), paths taken per queue pop (1-1024), tiles
  * accumulated per pass in LDS (16 or 32: fewer tiles, more waves per CU),
@@ -189,6 +211,17 @@ int ochre_context_set_batched_params(ochre_context* ctx, int waves_per_cu, int p
                                      int use_graph);
 int ochre_context_get_batched_params(ochre_context* ctx, int* waves_per_cu, int* paths_per_pop, int* tiles_per_chunk,
                                      int* use_graph);
+
+/* The dense single-walk kernel (on by default) runs in front of the ordered
+ * one and takes every path whose tile grid (line bounding box + a 2 px
+ * margin) has at most 64 tiles: one DDA walk adds each increment to its
+ * pixel's sums with LDS float atomics, so f32 sums are taken in arrival
+ * order and an alpha can differ from the ordered result by 1 LSB (tile and
+ * span sets are exact).  Off: every path goes through the ordered kernel,
+ * whose sums follow emission order.  groups_per_cu 1-8 resident workgroups,
+ * waves_per_group 2 or 4 (waves sharing one path). */
+int ochre_context_set_dense_params(ochre_context* ctx, int enabled, int groups_per_cu, int waves_per_group);
+int ochre_context_get_dense_params(ochre_context* ctx, int* enabled, int* groups_per_cu, int* waves_per_group);
 
 /* Coverage adjoint.  The loss of path p is the sum, over the pixels of the
  * union of its tiles and its target's tiles, of (cov - target)^2, where a
@@ -213,15 +246,20 @@ typedef struct ochre_coverage_target {
 int ochre_coverage_adjoint(ochre_context* ctx, const ochre_path_batch* in, const ochre_coverage_target* target,
                            float* loss, float* d_points, float* d_transforms, void* stream);
 
-/* Autotuner: times every combination of waves per CU {2,3,4,6,8,12},
- * paths per pop {1,2,4,8} and tiles per chunk {16,32} on the caller's batch
- * (`reps` timed calls each, median), keeps the fastest in the context and
- * reports every measurement in `results` (up to max_results entries). */
+/* Autotuner: with the dense kernel on, times every (workgroups per CU
+ * {1,2,3,4}, waves per workgroup {2,4}) of the dense launch; with it off,
+ * every combination of waves per CU {2,3,4,6,8,12}, paths per pop {1,2,4,8}
+ * and tiles per chunk {16,32} of the ordered launch.  Each configuration is
+ * timed on the caller's batch (`reps` calls, median); the fastest is kept in
+ * the context and every measurement reported in `results` (up to
+ * max_results entries). */
 typedef struct ochre_batched_tuning {
     int waves_per_cu;
     int paths_per_pop;
     int tiles_per_chunk;
     float ms;
+    int dense_groups_per_cu; /* 0: the dense kernel was off */
+    int dense_waves_per_group;
 } ochre_batched_tuning;
 int ochre_autotune_batched(ochre_context* ctx, const ochre_path_batch* in, ochre_batch_output* out, void* stream,
                            int reps, ochre_batched_tuning* results, int max_results, int* n_results);

@@ -16,6 +16,11 @@
 //   finalize  row carry (rasterizer.rs:233-250) and quantisation, 64 B stores
 // Paths too large for a wave (grid > 32x32 tiles or > 8192 lines) are
 // deferred to the plain pipeline by the host driver.
+//
+// With the dense mode on (the default), k_dense (dense.inc) runs first and
+// takes every path whose tile grid fits its LDS grid in a single walk with
+// order-independent accumulation; k_fused then works through the paths it
+// handed on.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -46,6 +51,10 @@ struct FusedIn {
     const int64_t* weight_off;
     const Xform* xf;
     int64_t n_points, n_weights;
+    // when set, the queue runs over ids[0 .. *n_ids) (paths handed on by
+    // k_dense) instead of 0 .. n_paths
+    const uint32_t* ids = nullptr;
+    const uint32_t* n_ids = nullptr;
 };
 
 struct FusedOut {
@@ -63,13 +72,14 @@ struct FusedOut {
 };
 
 struct FusedCtl {
-    uint32_t queue;
-    uint32_t deferred;
+    uint32_t queue;     // path queue of the first kernel of the launch
+    uint32_t deferred;  // paths handed to the plain pipeline
     uint32_t error;
-    uint32_t pad0;
+    uint32_t dq;        // paths handed on by k_dense
     unsigned long long tiles;
     unsigned long long spans;
-    uint32_t pad[56];
+    uint32_t queue2;    // queue of k_fused over the handed-on paths
+    uint32_t pad[55];
 };
 
 __global__ void k_fused_reset(FusedCtl* c) {
@@ -77,8 +87,10 @@ __global__ void k_fused_reset(FusedCtl* c) {
         c->queue = 0;
         c->deferred = 0;
         c->error = 0;
+        c->dq = 0;
         c->tiles = 0;
         c->spans = 0;
+        c->queue2 = 0;
     }
 }
 
@@ -697,17 +709,21 @@ __global__ __launch_bounds__(WAVE) void k_fused(FusedIn in, FusedOut out, FusedC
                                                 uint8_t* scratch, uint32_t gran) {
     __shared__ LdsState<TC> s;
     uint8_t* sc = scratch + (size_t)blockIdx.x * F_SCRATCH;
+    const int64_t n = in.ids ? (int64_t)*in.n_ids : in.n_paths;
     for (;;) {
         uint32_t base = 0;
-        if (threadIdx.x == 0) base = atomicAdd(&ctl->queue, gran);
+        if (threadIdx.x == 0) base = atomicAdd(in.ids ? &ctl->queue2 : &ctl->queue, gran);
         base = __shfl(base, 0, WAVE);
-        if (base >= (uint64_t)in.n_paths) break;
-        const int64_t end = std::min<int64_t>((int64_t)base + gran, in.n_paths);
-        for (int64_t p = base; p < end; p++) fused_path<TC, false>(in, out, ctl, deferred, sc, s, p, AdjArgs{}, nullptr);
+        if ((int64_t)base >= n) break;
+        const int64_t end = std::min<int64_t>((int64_t)base + gran, n);
+        for (int64_t q = base; q < end; q++)
+            fused_path<TC, false>(in, out, ctl, deferred, sc, s, in.ids ? (int64_t)in.ids[q] : q, AdjArgs{},
+                                  nullptr);
     }
 }
 
 #include "adjoint.inc"
+#include "dense.inc"
 
 // ---------------------------------------------- deferred paths -> plain
 
@@ -740,7 +756,7 @@ __global__ void k_scatter_ranges(const uint32_t* ids, uint32_t n, const int64_t*
 }
 
 struct BatchedState {
-    DBuf scratch, ascratch, ctl, deferred;
+    DBuf scratch, ascratch, ctl, deferred, dscratch, dense_ids;
     DBuf g_verbs, g_points, g_weights, g_voff, g_poff, g_woff, g_xf, g_ts, g_tc, g_ss, g_sc;
     int n_waves = 0;
     uint32_t gran = 1;
@@ -798,25 +814,54 @@ int run_batched(Context* ctx, const SegInput& in, OutputDesc& out, hipStream_t s
     uint8_t* scratch = B->scratch.as<uint8_t>();
     const uint32_t g = (uint32_t)std::max(1, gran);
     const bool tc16 = tiles_per_chunk <= 16;
-
-    auto enqueue = [&](hipStream_t st) {
-        hipLaunchKernelGGL(k_fused_reset, dim3(1), dim3(64), 0, st, ctl);
-        if (tc16) hipLaunchKernelGGL(k_fused<16>, dim3(n_waves), dim3(WAVE), 0, st, fi, fo, ctl, deferred, scratch, g);
-        else hipLaunchKernelGGL(k_fused<32>, dim3(n_waves), dim3(WAVE), 0, st, fi, fo, ctl, deferred, scratch, g);
-    };
+    const bool dense = ctx->dense;
+    const int dnw = ctx->dense_waves == 2 ? 2 : 4;
+    const int n_groups = std::max(1, cus * std::max(1, ctx->dense_groups_per_cu));
+    uint8_t* dscratch = nullptr;
+    uint32_t* dense_ids = nullptr;
+    FusedIn fi2 = fi;
+    if (dense) {
+        HIPCHK_RET(B->dscratch.ensure((size_t)n_groups * D_SCRATCH));
+        HIPCHK_RET(B->dense_ids.ensure((size_t)NP * 4));
+        dscratch = B->dscratch.as<uint8_t>();
+        dense_ids = B->dense_ids.as<uint32_t>();
+        fi2.ids = dense_ids;
+        fi2.n_ids = &ctl->dq;
+    }
     const bool timing = ctx->timing;
     if (timing) {
-        if (!ctx->ev[0]) HIPCHK_RET(hipEventCreate(&ctx->ev[0]));
-        if (!ctx->ev[1]) HIPCHK_RET(hipEventCreate(&ctx->ev[1]));
+        for (auto& e : ctx->ev)
+            if (!e) HIPCHK_RET(hipEventCreate(&e));
     }
+    // timing without a graph brackets each kernel with events
+    auto enqueue = [&](hipStream_t st, bool ev) {
+        hipLaunchKernelGGL(k_fused_reset, dim3(1), dim3(64), 0, st, ctl);
+        if (ev) (void)hipEventRecord(ctx->ev[0], st);
+        if (dense) {
+            if (dnw == 2)
+                hipLaunchKernelGGL(k_dense<2>, dim3(n_groups), dim3(2 * WAVE), 0, st, fi, fo, ctl, dense_ids,
+                                   dscratch);
+            else
+                hipLaunchKernelGGL(k_dense<4>, dim3(n_groups), dim3(4 * WAVE), 0, st, fi, fo, ctl, dense_ids,
+                                   dscratch);
+            if (ev) (void)hipEventRecord(ctx->ev[2], st);
+        }
+        const FusedIn& f = dense ? fi2 : fi;
+        if (tc16) hipLaunchKernelGGL(k_fused<16>, dim3(n_waves), dim3(WAVE), 0, st, f, fo, ctl, deferred, scratch, g);
+        else hipLaunchKernelGGL(k_fused<32>, dim3(n_waves), dim3(WAVE), 0, st, f, fo, ctl, deferred, scratch, g);
+        if (ev) (void)hipEventRecord(ctx->ev[1], st);
+    };
     if (use_graph) {
-        std::vector<uint8_t> key(sizeof(fi) + sizeof(fo) + 17);
+        std::vector<uint8_t> key(sizeof(fi) + sizeof(fo) + 33);
         key.back() = tc16 ? 1 : 0;
         memcpy(key.data(), &fi, sizeof(fi));
         memcpy(key.data() + sizeof(fi), &fo, sizeof(fo));
         memcpy(key.data() + sizeof(fi) + sizeof(fo), &n_waves, 4);
         memcpy(key.data() + sizeof(fi) + sizeof(fo) + 4, &g, 4);
         memcpy(key.data() + sizeof(fi) + sizeof(fo) + 8, &scratch, 8);
+        memcpy(key.data() + sizeof(fi) + sizeof(fo) + 16, &dscratch, 8);
+        const int dkey = dense ? (dnw << 16) | n_groups : 0;
+        memcpy(key.data() + sizeof(fi) + sizeof(fo) + 24, &dkey, 4);
         if (!B->exec || key != B->graph_key) {
             if (B->exec) (void)hipGraphExecDestroy(B->exec);
             if (B->graph) (void)hipGraphDestroy(B->graph);
@@ -825,7 +870,7 @@ int run_batched(Context* ctx, const SegInput& in, OutputDesc& out, hipStream_t s
             hipStream_t cs;
             HIPCHK_RET(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
             HIPCHK_RET(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-            enqueue(cs);
+            enqueue(cs, false);
             hipError_t e = hipStreamEndCapture(cs, &B->graph);
             (void)hipStreamDestroy(cs);
             HIPCHK_RET(e);
@@ -836,22 +881,35 @@ int run_batched(Context* ctx, const SegInput& in, OutputDesc& out, hipStream_t s
         HIPCHK_RET(hipGraphLaunch(B->exec, s));
         if (timing) HIPCHK_RET(hipEventRecord(ctx->ev[1], s));
     } else {
-        hipLaunchKernelGGL(k_fused_reset, dim3(1), dim3(64), 0, s, ctl);
-        if (timing) HIPCHK_RET(hipEventRecord(ctx->ev[0], s));
-        if (tc16) hipLaunchKernelGGL(k_fused<16>, dim3(n_waves), dim3(WAVE), 0, s, fi, fo, ctl, deferred, scratch, g);
-        else hipLaunchKernelGGL(k_fused<32>, dim3(n_waves), dim3(WAVE), 0, s, fi, fo, ctl, deferred, scratch, g);
-        if (timing) HIPCHK_RET(hipEventRecord(ctx->ev[1], s));
+        enqueue(s, timing);
     }
     HIPCHK_RET(hipGetLastError());
     FusedCtl* hc = reinterpret_cast<FusedCtl*>(reinterpret_cast<uint8_t*>(ctx->host_pinned) + 1024);
     HIPCHK_RET(hipMemcpyAsync(hc, ctl, sizeof(FusedCtl), hipMemcpyDeviceToHost, s));
     HIPCHK_RET(hipStreamSynchronize(s));
     if (timing) {
+        // graph: the whole launch; direct: k_dense and k_fused separately
         float ms = 0;
-        HIPCHK_RET(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
-        auto& t = ctx->timings["k_fused"];
-        t.first += ms;
-        t.second += 1;
+        if (use_graph) {
+            HIPCHK_RET(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+            auto& t = ctx->timings["batched_graph"];
+            t.first += ms;
+            t.second += 1;
+        } else if (dense) {
+            HIPCHK_RET(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[2]));
+            auto& t = ctx->timings["k_dense"];
+            t.first += ms;
+            t.second += 1;
+            HIPCHK_RET(hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[1]));
+            auto& t2 = ctx->timings["k_fused"];
+            t2.first += ms;
+            t2.second += 1;
+        } else {
+            HIPCHK_RET(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+            auto& t = ctx->timings["k_fused"];
+            t.first += ms;
+            t.second += 1;
+        }
     }
     if (hc->error & ERR_INPUT) return -1;
     if (hc->error & ERR_FLATTEN) return -5;

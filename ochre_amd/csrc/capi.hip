@@ -211,6 +211,18 @@ int ochre_stroke_batched(ochre_context* ctx, const ochre_path_batch* in, const f
     return stroke_batch(ctx, in, widths, out, stream, true);
 }
 
+int ochre_build_atlas(ochre_context* ctx, const ochre_batch_output* in, int64_t n_paths, const uint8_t* colors,
+                      uint8_t* atlas, int atlas_size, ochre_atlas_vertex* vertices, int64_t vertex_capacity,
+                      uint32_t* indices, int64_t index_capacity, int64_t* n_vertices, int64_t* n_indices,
+                      void* stream) {
+    if (!ctx || !in || !n_vertices || !n_indices) return OCHRE_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (hipSetDevice(ctx->c.device) != hipSuccess) return OCHRE_ERR_DEVICE;
+    OutputDesc od = to_output(in);
+    return run_atlas(&ctx->c, od, n_paths, colors, atlas, atlas_size, vertices, vertex_capacity, indices,
+                     index_capacity, n_vertices, n_indices, reinterpret_cast<hipStream_t>(stream));
+}
+
 int ochre_context_set_batched_params(ochre_context* ctx, int waves_per_cu, int paths_per_pop, int tiles_per_chunk,
                                      int use_graph) {
     if (!ctx || waves_per_cu < 1 || waves_per_cu > 32 || paths_per_pop < 1 || paths_per_pop > 1024 ||
@@ -232,6 +244,25 @@ int ochre_context_get_batched_params(ochre_context* ctx, int* waves_per_cu, int*
     if (paths_per_pop) *paths_per_pop = ctx->c.batched_gran;
     if (tiles_per_chunk) *tiles_per_chunk = ctx->c.batched_tc;
     if (use_graph) *use_graph = ctx->c.batched_graph ? 1 : 0;
+    return OCHRE_OK;
+}
+
+int ochre_context_set_dense_params(ochre_context* ctx, int enabled, int groups_per_cu, int waves_per_group) {
+    if (!ctx || groups_per_cu < 1 || groups_per_cu > 8 || (waves_per_group != 2 && waves_per_group != 4))
+        return OCHRE_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->c.dense = enabled != 0;
+    ctx->c.dense_groups_per_cu = groups_per_cu;
+    ctx->c.dense_waves = waves_per_group;
+    return OCHRE_OK;
+}
+
+int ochre_context_get_dense_params(ochre_context* ctx, int* enabled, int* groups_per_cu, int* waves_per_group) {
+    if (!ctx) return OCHRE_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (enabled) *enabled = ctx->c.dense ? 1 : 0;
+    if (groups_per_cu) *groups_per_cu = ctx->c.dense_groups_per_cu;
+    if (waves_per_group) *waves_per_group = ctx->c.dense_waves;
     return OCHRE_OK;
 }
 
@@ -270,28 +301,62 @@ int ochre_autotune_batched(ochre_context* ctx, const ochre_path_batch* in, ochre
     static const int waves[] = {2, 3, 4, 6, 8, 12};
     static const int grans[] = {1, 2, 4, 8};
     static const int tcs[] = {16, 32};
-    ochre_batched_tuning best{ctx->c.batched_waves_per_cu, ctx->c.batched_gran, ctx->c.batched_tc, 1e30f};
+    static const int dgroups[] = {1, 2, 3, 4};
+    static const int dwaves[] = {2, 4};
+    ochre_batched_tuning best{ctx->c.batched_waves_per_cu, ctx->c.batched_gran, ctx->c.batched_tc, 1e30f, 0, 0};
+    int best_dg = ctx->c.dense_groups_per_cu, best_dw = ctx->c.dense_waves;
     int k = 0;
-    for (int w : waves)
-        for (int g : grans)
-            for (int tc : tcs) {
-                OutputDesc od = to_output(out);
-                // warm-up (also captures the graph for this configuration)
-                st = run_batched(&ctx->c, si, od, s, w, g, tc, true);
+    auto measure = [&](int w, int g, int tc, float* ms_out) -> int {
+        OutputDesc od = to_output(out);
+        // warm-up (also captures the graph for this configuration)
+        int rc = run_batched(&ctx->c, si, od, s, w, g, tc, true);
+        if (rc) return rc;
+        std::vector<double> t;
+        for (int r = 0; r < reps; r++) {
+            auto t0 = std::chrono::steady_clock::now();
+            rc = run_batched(&ctx->c, si, od, s, w, g, tc, true);
+            if (rc) return rc;
+            t.push_back(std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+        }
+        std::sort(t.begin(), t.end());
+        *ms_out = (float)t[t.size() / 2];
+        return 0;
+    };
+    if (ctx->c.dense) {
+        // the ordered kernel only sees the paths the dense one hands on:
+        // tune the dense launch shape, with waves/CU and tiles per chunk
+        // encoded as (groups per CU, waves per group) in the results
+        for (int dg : dgroups)
+            for (int dw : dwaves) {
+                ctx->c.dense_groups_per_cu = dg;
+                ctx->c.dense_waves = dw;
+                float ms = 0;
+                st = measure(best.waves_per_cu, best.paths_per_pop, best.tiles_per_chunk, &ms);
                 if (st) return st;
-                std::vector<double> t;
-                for (int r = 0; r < reps; r++) {
-                    auto t0 = std::chrono::steady_clock::now();
-                    st = run_batched(&ctx->c, si, od, s, w, g, tc, true);
-                    if (st) return st;
-                    t.push_back(std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
-                }
-                std::sort(t.begin(), t.end());
-                const float ms = (float)t[t.size() / 2];
-                if (results && k < max_results) results[k] = ochre_batched_tuning{w, g, tc, ms};
+                if (results && k < max_results)
+                    results[k] = ochre_batched_tuning{best.waves_per_cu, best.paths_per_pop, best.tiles_per_chunk, ms,
+                                                      dg, dw};
                 k++;
-                if (ms < best.ms) best = ochre_batched_tuning{w, g, tc, ms};
+                if (ms < best.ms) {
+                    best.ms = ms;
+                    best_dg = dg;
+                    best_dw = dw;
+                }
             }
+        ctx->c.dense_groups_per_cu = best_dg;
+        ctx->c.dense_waves = best_dw;
+    } else {
+        for (int w : waves)
+            for (int g : grans)
+                for (int tc : tcs) {
+                    float ms = 0;
+                    st = measure(w, g, tc, &ms);
+                    if (st) return st;
+                    if (results && k < max_results) results[k] = ochre_batched_tuning{w, g, tc, ms, 0, 0};
+                    k++;
+                    if (ms < best.ms) best = ochre_batched_tuning{w, g, tc, ms, 0, 0};
+                }
+    }
     if (n_results) *n_results = k;
     ctx->c.batched_waves_per_cu = best.waves_per_cu;
     ctx->c.batched_gran = best.paths_per_pop;

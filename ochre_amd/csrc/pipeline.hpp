@@ -94,6 +94,13 @@ struct StrokeBuffers {
     ~StrokeBuffers();
 };
 
+// Device buffers of the atlas builder (per-path tile and span bases).
+struct AtlasState {
+    DBuf tile_base, span_base;
+    void* host = nullptr;  // pinned: totals read back
+    ~AtlasState();
+};
+
 struct Context {
     int device = 0;
     hipStream_t own_stream = nullptr;
@@ -112,14 +119,20 @@ struct Context {
     // timing
     bool timing = false;
     std::map<std::string, std::pair<double, int64_t>> timings;
-    hipEvent_t ev[2] = {nullptr, nullptr};
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     // batched-path state lives in batched.hip
     void* batched = nullptr;
     int batched_waves_per_cu = 6;
     int batched_gran = 2;
     int batched_tc = 32;
     bool batched_graph = true;
+    // dense single-walk kernel in front of the ordered one: on/off, resident
+    // workgroups per CU, waves per workgroup (2 or 4)
+    bool dense = true;
+    int dense_groups_per_cu = 4;
+    int dense_waves = 4;
     StrokeBuffers stroke;
+    AtlasState atlas;
     ~Context();
 };
 
@@ -157,6 +170,15 @@ int stroke_expand(Context* ctx, const SegInput& in, const float* widths, StrokeO
 // stroke_expand, then the plain or batched fill of the outlines.
 int run_stroke(Context* ctx, const SegInput& in, const float* widths, OutputDesc& out, hipStream_t stream,
                bool batched);
+
+}  // namespace ochre
+struct ochre_atlas_vertex;
+namespace ochre {
+// The demo's atlas Builder (examples/svg.rs:15-88) over a batch's device
+// output, paths in index order; colours [n_paths][4] device memory.
+int run_atlas(Context* ctx, const OutputDesc& in, int64_t n_paths, const uint8_t* colors, uint8_t* atlas,
+              int atlas_size, ochre_atlas_vertex* verts, int64_t vcap, uint32_t* idx, int64_t icap, int64_t* n_verts,
+              int64_t* n_idx, hipStream_t stream);
 
 #define HIPCHK_RET(x)                                      \
     do {                                                   \

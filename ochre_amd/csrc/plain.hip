@@ -48,6 +48,10 @@ DBuf::~DBuf() {
     if (p) (void)hipFree(p);
 }
 
+AtlasState::~AtlasState() {
+    if (host) (void)hipHostFree(host);
+}
+
 StrokeBuffers::~StrokeBuffers() {
     if (host) (void)hipHostFree(host);
 }

@@ -200,11 +200,40 @@ def stroke(ctx: Context, db: DeviceBatch, widths, out: DeviceOutput, batched=Tru
     return co.n_tiles, co.n_spans
 
 
+ATLAS_VERTEX_BYTES = 12  # i16 pos[2], u16 uv[2], u8 col[4] (examples/svg.rs:15-20)
+
+
+def build_atlas(ctx: Context, out: DeviceOutput, colors, atlas_size=4096, stream=None):
+    """The demo's atlas Builder on the device output of a batch (paths in
+    index order).  colors: uint8 tensor [n_paths, 4] on the device.  Returns
+    (atlas uint8 [S, S], vertices uint8 [n, 12], indices int32 [m]) tensors;
+    indices hold u32 values."""
+    torch = _torch()
+    co = out.c()
+    nv, ni = ctypes.c_int64(), ctypes.c_int64()
+    cp = colors.data_ptr() if colors.numel() else None
+    st = ctx.lib.ochre_build_atlas(ctx.h, ctypes.byref(co), out.n_paths, cp, None, atlas_size, None, 0, None, 0,
+                                   ctypes.byref(nv), ctypes.byref(ni), _stream_ptr(stream))
+    if st not in (0, -4):
+        _lib.check(st, "ochre_build_atlas")
+    atlas = torch.empty((atlas_size, atlas_size), dtype=torch.uint8, device="cuda")
+    verts = torch.empty((max(nv.value, 1), ATLAS_VERTEX_BYTES), dtype=torch.uint8, device="cuda")
+    idx = torch.empty((max(ni.value, 1),), dtype=torch.int32, device="cuda")
+    _lib.check(ctx.lib.ochre_build_atlas(ctx.h, ctypes.byref(co), out.n_paths, cp, atlas.data_ptr(), atlas_size,
+                                         verts.data_ptr(), nv.value, idx.data_ptr(), ni.value, ctypes.byref(nv),
+                                         ctypes.byref(ni), _stream_ptr(stream)), "ochre_build_atlas")
+    return atlas, verts[:nv.value], idx[:ni.value]
+
+
 def run_to_host(ctx, batch, mode="plain", tile_capacity=None, span_capacity=None):
     """Convenience: upload, rasterise (growing the outputs once if needed)
-    and return the per-path host arrays.  Modes: "plain", "batched",
+    and return the per-path host arrays.  Modes: "plain", "batched" (the
+    ordered batched kernel alone), "dense" (the dense kernel in front of it),
     "stroke_plain", "stroke_batched" (strokes use batch.widths)."""
     torch = _torch()
+    if mode in ("batched", "dense"):
+        d = dense_params(ctx)
+        set_dense_params(ctx, mode == "dense", d["dense_groups_per_cu"], d["dense_waves_per_group"])
     db = upload(batch)
     tc = tile_capacity or max(1024, batch.n_paths * 64)
     sc = span_capacity or max(1024, batch.n_paths * 8)
@@ -230,7 +259,23 @@ def batched_params(ctx: Context):
     w, g, tc, gr = (ctypes.c_int() for _ in range(4))
     _lib.check(ctx.lib.ochre_context_get_batched_params(ctx.h, ctypes.byref(w), ctypes.byref(g), ctypes.byref(tc),
                                                         ctypes.byref(gr)), "get_batched_params")
-    return {"waves_per_cu": w.value, "paths_per_pop": g.value, "tiles_per_chunk": tc.value, "graph": gr.value}
+    d = {"waves_per_cu": w.value, "paths_per_pop": g.value, "tiles_per_chunk": tc.value, "graph": gr.value}
+    d.update(dense_params(ctx))
+    return d
+
+
+def dense_params(ctx: Context):
+    en, gpc, wpg = (ctypes.c_int() for _ in range(3))
+    _lib.check(ctx.lib.ochre_context_get_dense_params(ctx.h, ctypes.byref(en), ctypes.byref(gpc), ctypes.byref(wpg)),
+               "get_dense_params")
+    return {"dense": en.value, "dense_groups_per_cu": gpc.value, "dense_waves_per_group": wpg.value}
+
+
+def set_dense_params(ctx: Context, enabled=True, groups_per_cu=4, waves_per_group=4):
+    """The dense single-walk kernel (order-independent f32 sums, alpha within
+    1 LSB of the ordered result) in front of the ordered kernel."""
+    _lib.check(ctx.lib.ochre_context_set_dense_params(ctx.h, 1 if enabled else 0, groups_per_cu, waves_per_group),
+               "set_dense_params")
 
 
 def set_batched_params(ctx: Context, waves_per_cu, paths_per_pop, tiles_per_chunk, graph=True):
@@ -252,6 +297,9 @@ def autotune(ctx: Context, db: DeviceBatch, out: DeviceOutput, reps=3, cache_key
             cached = None
         if cached:
             set_batched_params(ctx, cached["waves_per_cu"], cached["paths_per_pop"], cached["tiles_per_chunk"])
+            if "dense" in cached:
+                set_dense_params(ctx, bool(cached["dense"]), cached["dense_groups_per_cu"],
+                                 cached["dense_waves_per_group"])
             return cached, []
     res = (_lib.BatchedTuningC * 64)()
     n = ctypes.c_int()
@@ -259,6 +307,7 @@ def autotune(ctx: Context, db: DeviceBatch, out: DeviceOutput, reps=3, cache_key
     _lib.check(ctx.lib.ochre_autotune_batched(ctx.h, ctypes.byref(cb), ctypes.byref(co), _stream_ptr(stream), reps,
                                               res, 64, ctypes.byref(n)), "ochre_autotune_batched")
     allm = [{"waves_per_cu": r.waves_per_cu, "paths_per_pop": r.paths_per_pop, "tiles_per_chunk": r.tiles_per_chunk,
+             "dense_groups_per_cu": r.dense_groups_per_cu, "dense_waves_per_group": r.dense_waves_per_group,
              "ms": r.ms} for r in res[:n.value]]
     best = dict(batched_params(ctx))
     best["ms"] = min(m["ms"] for m in allm) if allm else None

@@ -101,6 +101,18 @@ int orc_stroke_path(const uint8_t* verbs, size_t n_verbs, const float* points, s
                     float width, orc_path* out);
 void orc_path_free(orc_path* p);
 
+/* The demo's atlas Builder (examples/svg.rs:15-88) fed by one Rasterizer
+ * per path in path order (svg.rs:140-156), colours [n_paths][4].  Clears the
+ * atlas (atlas_size^2 bytes), marks slot (0, 0) solid, then packs tiles from
+ * slot 1 on.  `verts` holds 12-byte vertices {i16 pos[2]; u16 uv[2];
+ * u8 col[4]}.  Returns the or of the per-path statuses, plus 0x100 when the
+ * atlas or the vertex/index capacity is exceeded (*n_verts / *n_idx still
+ * report the required counts). */
+int orc_atlas_batch(int64_t n_paths, const uint8_t* verbs, const int64_t* verb_off, const float* points,
+                    const int64_t* point_off, const float* weights, const int64_t* weight_off, const float* xforms,
+                    const float* widths, const uint8_t* colors, int atlas_size, uint8_t* atlas, void* verts,
+                    size_t vcap, uint32_t* idx, size_t icap, size_t* n_verts, size_t* n_idx);
+
 /* Batch runner: one Rasterizer per path (as examples/svg.rs:143-145 does),
  * fill (or stroke when widths != NULL) with the path's transform, finish. */
 typedef struct orc_result orc_result;

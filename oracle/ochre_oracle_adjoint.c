@@ -24,6 +24,13 @@
  *   points are world = M * local + offset (geom.rs:260-262).
  * Conic flattening (dyadic t, no dt) is differentiated through its rational
  * point formula; weights themselves are not differentiated.
+ * Discrete structure: how many segments a curve flattens to, and where a
+ * conic subdivides, is decided exactly as the reference decides it, in f32
+ * from the f32 world points (path.rs:50-57,63-73,86-103); the f64 forward
+ * then evaluates those segments' end points at t_k = k * dt in double
+ * precision.  The loss is therefore the f64 value of the reference's own
+ * piecewise-smooth coverage function, and its kinks (a segment count
+ * changing) sit where the reference's do.
  *
  * Parity pinning: the forward's u8 quantisation of cov matches the f32
  * oracle within 1 LSB (tests/test_oracle_adjoint.py); the adjoint is checked
@@ -127,6 +134,42 @@ static source src_point(int64_t g) {
 
 static d2 wpt(const d2* W, int64_t g) { return g < 0 ? mk(0, 0) : W[g]; }
 
+/* f32 mirror for the discrete decisions (path.rs:49-104 in f32) */
+typedef struct {
+    float x, y;
+} f2;
+static f2 fmk(float x, float y) { f2 r = {x, y}; return r; }
+static f2 fadd(f2 a, f2 b) { return fmk(a.x + b.x, a.y + b.y); }
+static f2 fsub(f2 a, f2 b) { return fmk(a.x - b.x, a.y - b.y); }
+static f2 fsmul(float s, f2 a) { return fmk(s * a.x, s * a.y); }
+static float flen(f2 a) { return sqrtf(a.x * a.x + a.y * a.y); }
+static f2 flerp(float t, f2 a, f2 b) { return fadd(fsmul(1.0f - t, a), fsmul(t, b)); }
+static f2 wpt32(const f2* W, int64_t g) { return g < 0 ? fmk(0.0f, 0.0f) : W[g]; }
+static float quad_dt32(f2 s, f2 c, f2 e) { return sqrtf((4.0f * 0.1f) / flen(fadd(fsub(s, fsmul(2.0f, c)), e))); }
+static float cubic_dt32(f2 s, f2 c1, f2 c2, f2 e) {
+    f2 a = fadd(fsub(fadd(fsmul(-1.0f, s), fsmul(3.0f, c1)), fsmul(3.0f, c2)), e);
+    f2 b = fsmul(3.0f, fadd(fsub(s, fsmul(2.0f, c1)), c2));
+    float conc = fmaxf(flen(b), flen(fadd(a, b)));
+    return sqrtf((sqrtf(8.0f) * 0.1f) / conc);
+}
+/* Number of segments of the reference's loop `while t < 1 { t = min(t + dt, 1) }`
+ * in f32 (0 when it would exceed the flattening limit). */
+static int steps32(float dt) {
+    float t = 0.0f;
+    int n = 0;
+    while (t < 1.0f) {
+        t = fminf(t + dt, 1.0f);
+        if (++n >= FLAT_LIMIT) return 0;
+    }
+    return n;
+}
+static f2 conic_at32(float t, f2 s, f2 c, f2 e, float w) {
+    f2 wc = fsmul(w, c);
+    f2 p01 = flerp(t, s, wc), p12 = flerp(t, wc, e);
+    float den = (1.0f - t) * (1.0f - t) + 2.0f * t * (1.0f - t) * w + t * t;
+    return fsmul(1.0f / den, flerp(t, p01, p12));
+}
+
 static double quad_dt(d2 s, d2 c, d2 e) { return sqrt((4.0 * TOL) / len(add(sub(s, smul(2.0, c)), e))); }
 static double cubic_dt(d2 s, d2 c1, d2 c2, d2 e) {
     d2 a = add(sub(add(smul(-1.0, s), smul(3.0, c1)), smul(3.0, c2)), e);
@@ -152,19 +195,24 @@ typedef struct {
     source prev_src;
     int64_t s, c, e;
     double w;
+    const f2* W32;
     int depth_err;
 } conic_state;
 
-static void conic_rec(conic_state* st, const d2* W, double t0, double t1, d2 p0, d2 p1, int depth) {
+/* flatten_conic (path.rs:75-104): subdivision decided in f32 (p0f, p1f are
+ * the f32 chord ends), points evaluated in f64 */
+static void conic_rec(conic_state* st, const d2* W, double t0, double t1, d2 p0, d2 p1, f2 p0f, f2 p1f,
+                      int depth) {
     double t = 0.5 * (t0 + t1);
     d2 S = wpt(W, st->s), C = W[st->c], E = W[st->e];
     d2 mid = conic_at(t, S, C, E, st->w);
-    double err = len(sub(mid, smul(0.5, add(p0, p1))));
-    if (err > TOL && depth < CONIC_DEPTH) {
-        conic_rec(st, W, t0, t, p0, mid, depth + 1);
-        conic_rec(st, W, t, t1, mid, p1, depth + 1);
+    const f2 midf = conic_at32((float)t, wpt32(st->W32, st->s), st->W32[st->c], st->W32[st->e], (float)st->w);
+    const float err = flen(fsub(midf, fsmul(0.5f, fadd(p0f, p1f))));
+    if (err > 0.1f && depth < CONIC_DEPTH) {
+        conic_rec(st, W, t0, t, p0, mid, p0f, midf, depth + 1);
+        conic_rec(st, W, t, t1, mid, p1, midf, p1f, depth + 1);
     } else {
-        if (err > TOL) st->depth_err = 1;
+        if (err > 0.1f) st->depth_err = 1;
         source sm;
         memset(&sm, 0, sizeof(sm));
         sm.type = SRC_CONIC;
@@ -185,8 +233,8 @@ static void conic_rec(conic_state* st, const d2* W, double t0, double t1, d2 p0,
 
 /* Lines fed to line_to for one path (rasterizer.rs:61-69,145-165,181-183),
  * in order, with the sources of their end points.  World points W[g]. */
-static int build_lines(const uint8_t* verbs, size_t nv, const d2* W, size_t np, const double* weights,
-                       size_t nw, lines* L) {
+static int build_lines(const uint8_t* verbs, size_t nv, const d2* W, const f2* W32, size_t np,
+                       const double* weights, size_t nw, lines* L) {
     int64_t g = 0, wi = 0;
     int64_t last = -1, first = -1; /* world point index of last / first (-1 origin) */
     d2 lastp = mk(0, 0), firstp = mk(0, 0);
@@ -207,13 +255,15 @@ static int build_lines(const uint8_t* verbs, size_t nv, const d2* W, size_t np, 
             break;
         case ORC_QUAD: {
             d2 S = lastp, C = W[g], E = W[g + 1];
+            const int n32 = steps32(quad_dt32(wpt32(W32, last), W32[g], W32[g + 1]));
+            if (n32 == 0) { L->status |= ORC_ERR_FLATTEN_LIMIT; }
             double dt = quad_dt(S, C, E), t = 0.0;
             int kk = 0;
             d2 prev = S;
             source ps = src_point(last);
-            while (t < 1.0) {
-                t = fmin(t + dt, 1.0);
+            while (kk < n32) {
                 kk++;
+                t = kk == n32 ? 1.0 : fmin((double)kk * dt, 1.0);
                 d2 q = quad_at(t, S, C, E);
                 source qs;
                 memset(&qs, 0, sizeof(qs));
@@ -238,13 +288,15 @@ static int build_lines(const uint8_t* verbs, size_t nv, const d2* W, size_t np, 
         }
         case ORC_CUBIC: {
             d2 S = lastp, C1 = W[g], C2 = W[g + 1], E = W[g + 2];
+            const int n32 = steps32(cubic_dt32(wpt32(W32, last), W32[g], W32[g + 1], W32[g + 2]));
+            if (n32 == 0) { L->status |= ORC_ERR_FLATTEN_LIMIT; }
             double dt = cubic_dt(S, C1, C2, E), t = 0.0;
             int kk = 0;
             d2 prev = S;
             source ps = src_point(last);
-            while (t < 1.0) {
-                t = fmin(t + dt, 1.0);
+            while (kk < n32) {
                 kk++;
+                t = kk == n32 ? 1.0 : fmin((double)kk * dt, 1.0);
                 d2 q = cubic_at(t, S, C1, C2, E);
                 source qs;
                 memset(&qs, 0, sizeof(qs));
@@ -278,8 +330,9 @@ static int build_lines(const uint8_t* verbs, size_t nv, const d2* W, size_t np, 
             st.c = g;
             st.e = g + 1;
             st.w = weights[wi++];
+            st.W32 = W32;
             st.depth_err = 0;
-            conic_rec(&st, W, 0.0, 1.0, lastp, W[g + 1], 0);
+            conic_rec(&st, W, 0.0, 1.0, lastp, W[g + 1], wpt32(W32, last), W32[g + 1], 0);
             if (st.depth_err) L->status |= ORC_ERR_FLATTEN_LIMIT;
             last = g + 1;
             lastp = W[g + 1];
@@ -664,13 +717,19 @@ int orc64_path_loss(const uint8_t* verbs, size_t nv, const double* points, size_
                     const int16_t* t_sxy, const uint16_t* t_sw, size_t t_ns, double* loss, double* dpts,
                     double* dxf, int16_t* out_xy, double* out_cov, size_t out_cap, size_t* out_nt) {
     d2* W = (d2*)malloc((np + 1) * sizeof(d2));
+    f2* W32 = (f2*)malloc((np + 1) * sizeof(f2));
+    const float X32[6] = {(float)xf[0], (float)xf[1], (float)xf[2], (float)xf[3], (float)xf[4], (float)xf[5]};
     for (size_t i = 0; i < np; i++) {
         double x = points[2 * i], y = points[2 * i + 1];
         W[i] = add(mk(xf[0] * x + xf[1] * y, xf[2] * x + xf[3] * y), mk(xf[4], xf[5]));
+        /* Transform::apply in f32 (geom.rs:170-179,260-262) */
+        const float xs = (float)x, ys = (float)y;
+        W32[i] = fadd(fmk(X32[0] * xs + X32[1] * ys, X32[2] * xs + X32[3] * ys), fmk(X32[4], X32[5]));
     }
     lines L;
     memset(&L, 0, sizeof(L));
-    int st = build_lines(verbs, nv, W, np, weights, nw, &L);
+    int st = build_lines(verbs, nv, W, W32, np, weights, nw, &L);
+    free(W32);
     if (st) { free(W); free(L.v); return st; }
     incrs I;
     memset(&I, 0, sizeof(I));

@@ -117,6 +117,40 @@ TILE_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int16, ctypes.c_int16
 SPAN_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int16, ctypes.c_int16, ctypes.c_uint16)
 
 
+ATLAS_VERTEX = np.dtype([("pos", "<i2", 2), ("uv", "<u2", 2), ("col", "u1", 4)])
+
+
+def atlas_batch(batch, colors, atlas_size=4096, stroke=False):
+    """The demo's atlas Builder (examples/svg.rs:15-88) fed by one oracle
+    Rasterizer per path, in path order.  Returns (status, atlas [S, S] u8,
+    vertices (ATLAS_VERTEX), indices u32)."""
+    lib = load()
+    fn = lib.orc_atlas_batch
+    fn.restype = ctypes.c_int
+    P = ctypes.c_void_p
+    fn.argtypes = [ctypes.c_int64, P, P, P, P, P, P, P, P, P, ctypes.c_int, P, P, ctypes.c_size_t, P,
+                   ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]
+    v = np.ascontiguousarray(batch.verbs, np.uint8)
+    vo = np.ascontiguousarray(batch.verb_off, np.int64)
+    p = np.ascontiguousarray(batch.points, np.float32)
+    po = np.ascontiguousarray(batch.point_off, np.int64)
+    w = np.ascontiguousarray(batch.weights, np.float32)
+    wo = np.ascontiguousarray(batch.weight_off, np.int64)
+    xf = np.ascontiguousarray(batch.xforms, np.float32)
+    widths = np.ascontiguousarray(batch.widths, np.float32) if stroke else None
+    col = np.ascontiguousarray(colors, np.uint8).reshape(-1, 4)
+    atlas = np.empty((atlas_size, atlas_size), np.uint8)
+    nv, ni = ctypes.c_size_t(), ctypes.c_size_t()
+    args = [batch.n_paths, _ptr(v), _ptr(vo), _ptr(p), _ptr(po), _ptr(w), _ptr(wo), _ptr(xf),
+            _ptr(widths) if widths is not None else None, _ptr(col), atlas_size, _ptr(atlas)]
+    fn(*args, None, 0, None, 0, ctypes.byref(nv), ctypes.byref(ni))
+    verts = np.zeros(nv.value, ATLAS_VERTEX)
+    idx = np.zeros(ni.value, np.uint32)
+    st = fn(*args, _ptr(verts) if verts.size else None, nv.value, _ptr(idx) if idx.size else None, ni.value,
+            ctypes.byref(nv), ctypes.byref(ni))
+    return st, atlas, verts, idx
+
+
 class Rasterizer:
     """Oracle Rasterizer with the reference's method set (rasterizer.rs:48-268)."""
 

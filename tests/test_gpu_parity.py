@@ -1,5 +1,8 @@
-"""Parity of both device pipelines (plain five-kernel, batched persistent)
-with the oracle, through the C ABI."""
+"""Parity of the device pipelines with the oracle, through the C ABI:
+plain (five kernels), batched (ordered persistent kernel) and dense (the
+single-walk kernel with order-independent f32 sums in front of it).  Tile
+and span sets must be identical; alpha bit-exact for plain and batched and
+within 1 LSB for dense, whose bit-exact fraction is printed."""
 import numpy as np
 import pytest
 
@@ -10,7 +13,7 @@ from parity import compare
 pytestmark = pytest.mark.gpu
 
 
-MODES = ["plain", "batched"]
+MODES = ["plain", "batched", "dense"]
 
 
 def check(ctx, batch, mode, max_lsb=1, min_exact=None):
@@ -18,6 +21,11 @@ def check(ctx, batch, mode, max_lsb=1, min_exact=None):
     print(f"[{mode}] {batch.name} paths={batch.n_paths}", flush=True)
     ref = oracle_ref.run_batch(batch)
     got = run_to_host(ctx, batch, mode)
+    if mode == "dense":
+        # arrival-order f32 sums: 1 LSB where a sum sits on a quantisation
+        # boundary, which is rare
+        max_lsb = max(max_lsb, 1)
+        min_exact = max(min_exact or 0.0, 0.999)
     st = compare(ref, got, max_lsb=max_lsb)
     print(f"[{mode}] {batch.name}: {st}", flush=True)
     if min_exact is not None:
@@ -154,15 +162,23 @@ def test_batched_params_and_graph_replays(gpu_ctx):
     ref = oracle_ref.run_batch(batch)
     db = upload(batch)
     lib = gpu_ctx.lib
+    configs = [(0, 0, 0, 2, 1, 32, 1), (0, 0, 0, 6, 1, 16, 1), (0, 0, 0, 8, 4, 32, 1), (0, 0, 0, 4, 16, 16, 0),
+               (0, 0, 0, 1, 2, 32, 0), (1, 4, 4, 6, 2, 32, 1), (1, 2, 2, 6, 2, 32, 1), (1, 1, 4, 2, 1, 16, 0),
+               (1, 3, 4, 8, 1, 32, 1)]
     try:
-        for waves, gran, tc, graph in [(2, 1, 32, 1), (6, 1, 16, 1), (8, 4, 32, 1), (4, 16, 16, 0), (1, 2, 32, 0)]:
+        for dense, dg, dw, waves, gran, tc, graph in configs:
+            assert lib.ochre_context_set_dense_params(gpu_ctx.h, dense, max(dg, 1), dw or 4) == 0
             assert lib.ochre_context_set_batched_params(gpu_ctx.h, waves, gran, tc, graph) == 0
             for rep in range(3):
                 out = DeviceOutput(batch.n_paths, ref.n_tiles + 16, ref.n_spans + 16)
                 rasterize_batched(gpu_ctx, db, out)
                 torch.cuda.synchronize()
-                st = compare(ref, out.to_host(), max_lsb=0)
-                print(f"[batched] waves={waves} gran={gran} tc={tc} graph={graph} rep={rep}: {st}", flush=True)
+                st = compare(ref, out.to_host(), max_lsb=1 if dense else 0)
+                print(f"[batched] dense={dense}/{dg}/{dw} waves={waves} gran={gran} tc={tc} graph={graph} "
+                      f"rep={rep}: {st}", flush=True)
+                if dense:
+                    assert st["bit_exact_fraction"] >= 0.999
                 _ = np.sum(np.arange(1000))  # host work between replays
     finally:
         lib.ochre_context_set_batched_params(gpu_ctx.h, 6, 2, 32, 1)
+        lib.ochre_context_set_dense_params(gpu_ctx.h, 1, 4, 4)
