@@ -64,6 +64,65 @@ def cpu_baseline(batch, seconds):
             "sample": f"{batch.name}: all {batch.n_paths} paths, median of {reps} passes (single thread)"}, ref
 
 
+_ALL_CORES_BATCH = None
+
+
+def _range_batch(batch, a, b):
+    import numpy as np
+    from ochre_amd import workloads as W
+    vo, po, wo = batch.verb_off, batch.point_off, batch.weight_off
+    return W.PathBatch(batch.verbs[vo[a]:vo[b]], vo[a:b + 1] - vo[a], batch.points[po[a]:po[b]], po[a:b + 1] - po[a],
+                       batch.weights[wo[a]:wo[b]], wo[a:b + 1] - wo[a], np.ascontiguousarray(batch.xforms[a:b]))
+
+
+def _all_cores_worker(ab):
+    a, b = ab
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_ref
+    sub = _range_batch(_ALL_CORES_BATCH, a, b)
+    t = time.perf_counter()
+    oracle_ref.run_batch(sub)
+    return time.perf_counter() - t
+
+
+def cpu_all_cores(batch, passes=3):
+    """The oracle on every host core (one process per core, a contiguous
+    range of paths each, forked before any core pinning); the pass time is the
+    slowest worker's."""
+    import multiprocessing as mp
+    global _ALL_CORES_BATCH
+    n = max(1, min(len(os.sched_getaffinity(0)), 64))
+    _ALL_CORES_BATCH = batch
+    cuts = [round(batch.n_paths * k / n) for k in range(n + 1)]
+    ranges = [(cuts[k], cuts[k + 1]) for k in range(n) if cuts[k + 1] > cuts[k]]
+    best = None
+    with mp.get_context("fork").Pool(len(ranges)) as pool:
+        for _ in range(passes):
+            t = max(pool.map(_all_cores_worker, ranges))
+            best = t if best is None else min(best, t)
+    return {"value": batch.n_paths / best, "unit": "paths/s", "cores": len(ranges), "kind": "port",
+            "sample": f"{batch.name}: all {batch.n_paths} paths split over {len(ranges)} processes, best of {passes}"}
+
+
+def api_latency(reps=20):
+    """The drop-in Rasterizer API end to end on the README path: fill, finish
+    (upload, device pipeline, download) and the TileBuilder callbacks."""
+    from ochre_amd import EventRecorder, Rasterizer, workloads as W
+    b = W.readme_path()
+    v, p, w, xf = b.path(0)
+    times = []
+    for _ in range(reps + 3):
+        t = time.perf_counter()
+        r = Rasterizer()
+        r.fill_arrays(v, p, w, xf)
+        rec = EventRecorder()
+        r.finish(rec)
+        times.append(time.perf_counter() - t)
+    times = sorted(times[3:])
+    return {"ms": times[len(times) // 2] * 1e3, "workload": "readme_path (1 path, 100 tiles, 23 spans)",
+            "what": "Rasterizer.fill + finish through the C ABI, TileBuilder callbacks included, median"}
+
+
 def traffic_bytes(kernel, workload):
     """HBM bytes per launch of `kernel` on `workload` from the committed
     rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated), or
@@ -257,7 +316,12 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_bytes(res[head]["kernel"], batch.name),
                      "kernel": res[head]["kernel"], "kernel_ms": kms, "algorithmic_bytes": alg},
     }
+    line["rasterizer_api_latency"] = api_latency()
     if not args.no_cpu:
+        try:
+            line["cpu_baseline_all_cores"] = cpu_all_cores(batch)
+        except Exception as e:  # a restricted host: report the single-core figure only
+            log(f"all-core CPU baseline skipped: {e}")
         cb, ref = cpu_baseline(batch, args.cpu_seconds)
         cb["cpu"] = platform.processor() or platform.machine()
         line["cpu_baseline"] = cb

@@ -65,7 +65,11 @@ def test_adjoint_matches_f64_oracle_and_finite_differences(gpu_ctx, n, seed):
         p64, xf64 = p.astype(np.float64), xf.astype(np.float64)
         st, L, gp, gx, _ = O.path_loss64(v, p64, w, xf64, t, grad=True)
         assert st == 0
-        assert abs(loss[i] - L) <= 1e-4 * max(L, 1.0), (i, loss[i], L)
+        # f32 coverage against f64: at canvas coordinates up to 1024 px an f32
+        # point carries ~6e-5 px of rounding, so an edge pixel's coverage
+        # differs by up to ~1e-4 and the loss (hundreds of edge pixels, each
+        # d(cov - t)^2 = 2 (cov - t) dcov) by ~1e-3 relative at most
+        assert abs(loss[i] - L) <= 1e-3 * max(L, 1.0), (i, loss[i], L)
         a, e = b.point_off[i], b.point_off[i + 1]
         scale = max(np.abs(gp).max(), np.abs(gx).max(), 1.0)
         err = max(np.abs(dp[a:e] - gp).max(), np.abs(dx[i] - gx).max()) / scale
