@@ -25,6 +25,18 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "glyph paths/s + Mtiles/s on 1 MI355X (batched vs plain), % of HBM peak"
 
 
+WORKLOAD_INFO = {
+    "glyph_run": {"canvas": 4096, "segments": "1-4 contours x 8-40 (40% line, 40% quad, 20% cubic)",
+                  "size_px": "12-48"},
+    "star_blobs": {"canvas": 4096, "segments": "one contour of log-uniform 1k-100k cubics",
+                   "radius_px": "200-2000, noise 15%"},
+    "scenes": {"scenes": 256, "canvas": 2048, "paths_per_scene": "log-uniform 1-5000, 5% blobs of 1k-20k cubics",
+               "transform": "affine: scale 0.5-4, rotation, shear <= 0.2"},
+    "adjoint_glyphs": {"canvas": 1024, "segments": "16 cubics", "size_px": "32-64"},
+    "stroked_glyphs": {"canvas": 4096, "segments": "glyph geometry in px, stroked 0.75-3 px"},
+}
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -40,7 +52,10 @@ def algorithmic_bytes(batch, n_tiles, n_spans):
 
 def cpu_baseline(batch, seconds):
     """The oracle (a sequential C port of the reference Rasterizer) on one
-    host core, repeated over the same batch until `seconds` have elapsed."""
+    host core, repeated over the same batch until `seconds` have elapsed.  A
+    batch too large for that budget (more than 2M verbs) is sampled instead:
+    consecutive paths from the start, one at a time, until the budget is
+    spent (no oracle result is returned then)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_ref
     try:
@@ -48,12 +63,21 @@ def cpu_baseline(batch, seconds):
     except Exception:
         pass
     oracle_ref.load()
+    if batch.verbs.size > 2_000_000:
+        t0 = time.perf_counter()
+        done = 0
+        while done < batch.n_paths and (time.perf_counter() - t0 < seconds or done == 0):
+            oracle_ref.run_batch(_range_batch(batch, done, done + 1), stroke=batch.widths is not None)
+            done += 1
+        el = time.perf_counter() - t0
+        return {"value": done / el, "unit": "paths/s", "cores": 1, "kind": "port",
+                "sample": f"{batch.name}: the first {done} of {batch.n_paths} paths in {el:.1f} s (single thread)"}, None
     reps, t0 = 0, time.perf_counter()
     times = []
     ref = None
     while True:
         a = time.perf_counter()
-        ref = oracle_ref.run_batch(batch)
+        ref = oracle_ref.run_batch(batch, stroke=batch.widths is not None)
         times.append(time.perf_counter() - a)
         reps += 1
         if time.perf_counter() - t0 >= seconds and reps >= 2:
@@ -71,8 +95,12 @@ def _range_batch(batch, a, b):
     import numpy as np
     from ochre_amd import workloads as W
     vo, po, wo = batch.verb_off, batch.point_off, batch.weight_off
-    return W.PathBatch(batch.verbs[vo[a]:vo[b]], vo[a:b + 1] - vo[a], batch.points[po[a]:po[b]], po[a:b + 1] - po[a],
-                       batch.weights[wo[a]:wo[b]], wo[a:b + 1] - wo[a], np.ascontiguousarray(batch.xforms[a:b]))
+    sub = W.PathBatch(batch.verbs[vo[a]:vo[b]], vo[a:b + 1] - vo[a], batch.points[po[a]:po[b]],
+                      po[a:b + 1] - po[a], batch.weights[wo[a]:wo[b]], wo[a:b + 1] - wo[a],
+                      np.ascontiguousarray(batch.xforms[a:b]))
+    if batch.widths is not None:
+        sub.widths = np.ascontiguousarray(batch.widths[a:b])
+    return sub
 
 
 def _all_cores_worker(ab):
@@ -81,7 +109,7 @@ def _all_cores_worker(ab):
     import oracle_ref
     sub = _range_batch(_ALL_CORES_BATCH, a, b)
     t = time.perf_counter()
-    oracle_ref.run_batch(sub)
+    oracle_ref.run_batch(sub, stroke=sub.widths is not None)
     return time.perf_counter() - t
 
 
@@ -202,16 +230,25 @@ def main():
     ctx = Context(0)
     db = upload(batch)
     torch.cuda.synchronize()
+    stroking = batch.widths is not None
+    if stroking:  # Rasterizer::stroke for every path: expansion + fill on the device
+        from ochre_amd.device import stroke
+        wd = torch.from_numpy(batch.widths).to("cuda")
+        run_plain = lambda o: stroke(ctx, db, wd, o, batched=False)  # noqa: E731
+        run_batched = lambda o: stroke(ctx, db, wd, o, batched=True)  # noqa: E731
+    else:
+        run_plain = lambda o: rasterize_plain(ctx, db, o)  # noqa: E731
+        run_batched = lambda o: rasterize_batched(ctx, db, o)  # noqa: E731
 
     # size the outputs with one call
     out = DeviceOutput(batch.n_paths, batch.n_paths * 64, batch.n_paths * 8)
     try:
-        rasterize_plain(ctx, db, out)
+        run_plain(out)
     except Exception as e:  # capacity: grow once
         if getattr(e, "status", None) != -4:
             raise
         out = DeviceOutput(batch.n_paths, out.n_tiles, out.n_spans)
-        rasterize_plain(ctx, db, out)
+        run_plain(out)
     torch.cuda.synchronize()
     n_tiles, n_spans = out.n_tiles, out.n_spans
     b_in, b_out = algorithmic_bytes(batch, n_tiles, n_spans)
@@ -228,6 +265,8 @@ def main():
             set_batched_params(ctx, w, g, tc, graph=not args.no_graph)
             tuned = batched_params(ctx)
             log(f"batched parameters set: {tuned}")
+        elif stroking:
+            tuned = batched_params(ctx)  # the fill of the outlines runs with the defaults
         else:
             best, allm = autotune(ctx, db, out, reps=3)
             if args.no_graph:
@@ -245,7 +284,7 @@ def main():
     outputs = {}
     for mode in modes:
         if mode == "plain":
-            fn = lambda: rasterize_plain(ctx, db, out)  # noqa: E731
+            fn = lambda: run_plain(out)  # noqa: E731
             ctx.set_timing(True)
             ev_ms, wall_ms = time_steps(fn, args.steps, args.warmup)
             kms, kn = ctx.timing("k_tiles")
@@ -253,7 +292,7 @@ def main():
             kernels = {"k_tiles": kms / max(kn, 1)}
             kname = "k_tiles"
         else:
-            fn = lambda: rasterize_batched(ctx, db, out)  # noqa: E731
+            fn = lambda: run_batched(out)  # noqa: E731
             # the production launch (graph replay unless --no-graph)
             ev_ms, wall_ms = time_steps(fn, args.steps, args.warmup)
             # kernel durations: the same launches made directly, with HIP
@@ -300,9 +339,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": batch.name, "paths": batch.n_paths, "canvas": 4096,
-                   "segments": "1-4 contours x 8-40 (40% line, 40% quad, 20% cubic)", "size_px": "12-48",
-                   "pipeline": head},
+        "config": dict({"workload": batch.name, "paths": batch.n_paths, "pipeline": head},
+                       **WORKLOAD_INFO.get(args.workload, {})),
         "mtiles_per_s": n_tiles / ms * 1e-3,
         "tiles": n_tiles,
         "spans": n_spans,
@@ -327,8 +365,9 @@ def main():
         line["cpu_baseline"] = cb
         line["vs_cpu_1core"] = paths_s / cb["value"]
         # the oracle's result from the baseline leg checks every timed output
-        from parity import compare
-        line["parity"] = {m: compare(ref, o, max_lsb=1) for m, o in outputs.items()}
+        if ref is not None:
+            from parity import compare
+            line["parity"] = {m: compare(ref, o, max_lsb=1) for m, o in outputs.items()}
     else:
         line["cpu_baseline"] = None
     print(json.dumps(line), flush=True)

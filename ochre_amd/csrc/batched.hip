@@ -208,6 +208,30 @@ __device__ void adjoint_backward(const FusedIn& in, const AdjArgs& adj, FusedCtl
                                  LdsState<TC>& s, int64_t p, uint32_t L, uint32_t NT, uint32_t NS, int32_t tx0,
                                  int32_t ty0, int32_t GW, int32_t GH);
 
+// The adjoint evaluates its forward in path-local device coordinates: the
+// transform's offset less a tile-aligned origin (the first point's tile), so
+// that f32 coordinates stay small.  At 1000 px an f32 coordinate carries
+// ~6e-5 px of rounding, enough to move a pixel's |v| across the clamp at 1
+// relative to the exact coverage; local coordinates cut that tenfold.  A
+// whole-tile shift leaves the pixel and tile grid unchanged, and the
+// derivatives with respect to the offset are those of the original one.
+// (ox, oy) is the origin in tiles.
+__device__ inline Xform adjoint_local_xform(const FusedIn& in, int64_t p, int32_t* ox, int32_t* oy) {
+    Xform X = in.xf[p];
+    *ox = *oy = 0;
+    const int64_t pb = in.point_off[p];
+    if (in.point_off[p + 1] > pb) {
+        const V2 w = apply(X, mk(in.points[2 * pb], in.points[2 * pb + 1]));
+        if (fabsf(w.x) < 1.0e6f && fabsf(w.y) < 1.0e6f) {
+            *ox = (int32_t)floorf(w.x * 0.125f);
+            *oy = (int32_t)floorf(w.y * 0.125f);
+            X.ox -= (float)(*ox * TILE);
+            X.oy -= (float)(*oy * TILE);
+        }
+    }
+    return X;
+}
+
 template <int TC, bool ADJ>
 __device__ void fused_path(const FusedIn& in, const FusedOut& out, FusedCtl* ctl, uint32_t* deferred,
                            uint8_t* sc, LdsState<TC>& s, int64_t p, const AdjArgs& adj, uint8_t* asc) {
@@ -220,7 +244,8 @@ __device__ void fused_path(const FusedIn& in, const FusedOut& out, FusedCtl* ctl
     const int64_t vb = in.verb_off[p], ve = in.verb_off[p + 1];
     const int64_t pb = in.point_off[p], pe = in.point_off[p + 1];
     const int64_t wb = in.weight_off[p], we = in.weight_off[p + 1];
-    const Xform X = in.xf[p];
+    int32_t aox = 0, aoy = 0;
+    const Xform X = ADJ ? adjoint_local_xform(in, p, &aox, &aoy) : in.xf[p];
 
     // ---------------------------------------------------------- flatten
     uint32_t L = 0;
