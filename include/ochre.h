@@ -247,7 +247,8 @@ int ochre_coverage_adjoint(ochre_context* ctx, const ochre_path_batch* in, const
                            float* loss, float* d_points, float* d_transforms, void* stream);
 
 /* Autotuner: with the dense kernel on, times every (workgroups per CU
- * {1,2,3,4}, waves per workgroup {2,4}) of the dense launch; with it off,
+ * {1,2,3,4}, waves per workgroup {2,4}) of the dense launch, then waves per
+ * CU {1,2,4,8} of the ordered kernel behind it; with it off,
  * every combination of waves per CU {2,3,4,6,8,12}, paths per pop {1,2,4,8}
  * and tiles per chunk {16,32} of the ordered launch.  Each configuration is
  * timed on the caller's batch (`reps` calls, median); the fastest is kept in

@@ -345,6 +345,20 @@ int ochre_autotune_batched(ochre_context* ctx, const ochre_path_batch* in, ochre
             }
         ctx->c.dense_groups_per_cu = best_dg;
         ctx->c.dense_waves = best_dw;
+        // then the persistent grid of the ordered kernel behind it, which
+        // only works through the paths the dense kernel hands on
+        for (int w : {1, 2, 4, 8}) {
+            float ms = 0;
+            st = measure(w, best.paths_per_pop, best.tiles_per_chunk, &ms);
+            if (st) return st;
+            if (results && k < max_results)
+                results[k] = ochre_batched_tuning{w, best.paths_per_pop, best.tiles_per_chunk, ms, best_dg, best_dw};
+            k++;
+            if (ms < best.ms) {
+                best.ms = ms;
+                best.waves_per_cu = w;
+            }
+        }
     } else {
         for (int w : waves)
             for (int g : grans)

@@ -214,42 +214,75 @@ struct FlattenArgs {
 
 template <bool EMIT>
 __global__ __launch_bounds__(256) void k_flatten(FlattenArgs a) {
-    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i >= a.in.n_verbs) return;
-    Cmd c = resolve_cmd(a.in, a.scan, i);
-    if (c.bad) {
-        atomicOr(&a.cnt->error, (uint32_t)ERR_INPUT);
-        if (!EMIT) a.cmd_lines[i] = 0;
-        return;
-    }
-    if (!EMIT) {
-        uint32_t n = 0;
-        BoxAcc box;
-        bool ok = command_lines(c, [&](V2 p0, V2 p1) {
-            n++;
-            box.add(p0);
-            box.add(p1);
-        });
-        if (!ok) atomicOr(&a.cnt->error, (uint32_t)ERR_FLATTEN);
-        a.cmd_lines[i] = n;
-        if (n > 0) {
-            PathBox* b = &a.box[c.path];
-            if (box.full) atomicOr(&b->flags, (uint32_t)(BOX_FULL | BOX_LINES));
-            else atomicOr(&b->flags, (uint32_t)BOX_LINES);
-            if (box.minx <= box.maxx) {
-                atomicMin(&b->minx, (int32_t)floorf(box.minx));
-                atomicMin(&b->miny, (int32_t)floorf(box.miny));
-                atomicMax(&b->maxx, (int32_t)floorf(box.maxx));
-                atomicMax(&b->maxy, (int32_t)floorf(box.maxy));
-            }
-        }
-    } else {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const bool in_range = i < a.in.n_verbs;
+    if (EMIT) {
+        if (!in_range) return;
+        Cmd c = resolve_cmd(a.in, a.scan, i);
+        if (c.bad) return;
         uint32_t o = a.cmd_line_off[i];
         command_lines(c, [&](V2 p0, V2 p1) {
             a.lines[o] = make_float4(p0.x, p0.y, p1.x, p1.y);
             a.line_path[o] = (uint32_t)c.path;
             o++;
         });
+        return;
+    }
+    // count pass: lines per command, and the path's line bounding box.  A
+    // path's commands are contiguous, so the wave reduces the box per path
+    // (segmented by path) and one lane per path segment does the atomics.
+    int32_t key = -1;
+    uint32_t n = 0, flags = 0;
+    BoxAcc box;
+    if (in_range) {
+        Cmd c = resolve_cmd(a.in, a.scan, i);
+        key = c.path;
+        if (c.bad) {
+            atomicOr(&a.cnt->error, (uint32_t)ERR_INPUT);
+        } else {
+            const bool ok = command_lines(c, [&](V2 p0, V2 p1) {
+                n++;
+                box.add(p0);
+                box.add(p1);
+            });
+            if (!ok) atomicOr(&a.cnt->error, (uint32_t)ERR_FLATTEN);
+        }
+        a.cmd_lines[i] = n;
+        if (n > 0) flags = box.full ? (uint32_t)(BOX_FULL | BOX_LINES) : (uint32_t)BOX_LINES;
+    }
+    int32_t mnx = INT32_MAX, mny = INT32_MAX, mxx = INT32_MIN, mxy = INT32_MIN;
+    if (n > 0 && box.minx <= box.maxx) {
+        mnx = (int32_t)floorf(box.minx);
+        mny = (int32_t)floorf(box.miny);
+        mxx = (int32_t)floorf(box.maxx);
+        mxy = (int32_t)floorf(box.maxy);
+    }
+    const int lane = threadIdx.x & (WAVE - 1);
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) {
+        const int32_t k2 = __shfl_up(key, d, WAVE);
+        const uint32_t f2 = __shfl_up(flags, d, WAVE);
+        const int32_t a0 = __shfl_up(mnx, d, WAVE), a1 = __shfl_up(mny, d, WAVE);
+        const int32_t a2 = __shfl_up(mxx, d, WAVE), a3 = __shfl_up(mxy, d, WAVE);
+        if (lane >= d && k2 == key) {
+            flags |= f2;
+            mnx = min(mnx, a0);
+            mny = min(mny, a1);
+            mxx = max(mxx, a2);
+            mxy = max(mxy, a3);
+        }
+    }
+    const int32_t knext = __shfl_down(key, 1, WAVE);
+    const bool seg_end = lane == WAVE - 1 || knext != key;
+    if (key >= 0 && seg_end && flags) {
+        PathBox* b = &a.box[key];
+        atomicOr(&b->flags, flags);
+        if (mnx <= mxx) {
+            atomicMin(&b->minx, mnx);
+            atomicMin(&b->miny, mny);
+            atomicMax(&b->maxx, mxx);
+            atomicMax(&b->maxy, mxy);
+        }
     }
 }
 
