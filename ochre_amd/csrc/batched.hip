@@ -278,7 +278,10 @@ __device__ void fused_path(const FusedIn& in, const FusedOut& out, FusedCtl* ctl
         c.bad = false;
         if (valid && g + np > pe) lane_bad = true;
         if (valid && !lane_bad) {
-            for (int k = 0; k < np; k++) c.pt[k] = fload(in, g + k, X);
+            // static indices keep the command in registers
+            if (np > 0) c.pt[0] = fload(in, g, X);
+            if (np > 1) c.pt[1] = fload(in, g + 1, X);
+            if (np > 2) c.pt[2] = fload(in, g + 2, X);
             c.w = 0.0f;
             if (verb == CONIC) {
                 int64_t wi = wb + ccon + (cc_incl - 1);

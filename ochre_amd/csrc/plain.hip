@@ -178,7 +178,10 @@ __device__ inline Cmd resolve_cmd(const SegInput& in, const CmdAcc* scan, int64_
         c.bad = true;
         np = 0;
     }
-    for (int k = 0; k < np; k++) c.pt[k] = load_point(in, g + k, s);
+    // static indices keep the command in registers
+    if (np > 0) c.pt[0] = load_point(in, g, s);
+    if (np > 1) c.pt[1] = load_point(in, g + 1, s);
+    if (np > 2) c.pt[2] = load_point(in, g + 2, s);
     c.w = 0.0f;
     if (c.verb == CONIC) {
         if ((int64_t)a.conics < in.n_weights) c.w = in.weights[a.conics];

@@ -64,8 +64,10 @@ __device__ uint32_t flatten_one(const PathSrc& in, int64_t p, Emitter& out) {
         const int verb = in.verbs[i];
         const int k = verb_points(verb);
         if (k < 0 || g + k > ge || (verb == CONIC && wi >= we)) return ERR_INPUT;
-        V2 pt[3];
-        for (int j = 0; j < k; j++) pt[j] = mk(in.points[2 * (g + j)], in.points[2 * (g + j) + 1]);
+        V2 pt[3] = {mk(0.0f, 0.0f), mk(0.0f, 0.0f), mk(0.0f, 0.0f)};  // static indices: registers
+        if (k > 0) pt[0] = mk(in.points[2 * g], in.points[2 * g + 1]);
+        if (k > 1) pt[1] = mk(in.points[2 * (g + 1)], in.points[2 * (g + 1) + 1]);
+        if (k > 2) pt[2] = mk(in.points[2 * (g + 2)], in.points[2 * (g + 2) + 1]);
         if (verb == MOVE || verb == LINE) {
             out.push(verb, pt[0]);
         } else if (verb == QUAD || verb == CUBIC) {
@@ -83,7 +85,9 @@ __device__ uint32_t flatten_one(const PathSrc& in, int64_t p, Emitter& out) {
         } else {
             out.push(CLOSE, mk(0.0f, 0.0f));
         }
-        if (k > 0) last = pt[k - 1];
+        if (k == 1) last = pt[0];
+        else if (k == 2) last = pt[1];
+        else if (k == 3) last = pt[2];
         g += k;
     }
     return g == ge && wi == we ? 0u : (uint32_t)ERR_INPUT;

@@ -75,8 +75,13 @@ def test_adjoint_matches_f64_oracle_and_finite_differences(gpu_ctx, n, seed):
         err = max(np.abs(dp[a:e] - gp).max(), np.abs(dx[i] - gx).max()) / scale
         worst_g = max(worst_g, err)
         assert err <= 1e-3, (i, err)
-        # finite differences on one sampled parameter per path (64 total at n=64+)
-        if checked_fd < 64:
+        # central finite differences of the f64 forward (h = 1e-3 px) on
+        # sampled parameters, up to 3 tries per path for one without a kink
+        # in [x - h, x + h] (a kink: the one-sided quotients disagree by more
+        # than the curvature of a smooth loss explains at this h)
+        for _ in range(3):
+            if checked_fd >= 64:
+                break
             k = int(rng.integers(0, p64.size + 6))
             if k < p64.size:
                 h = 1e-3 / float(xf64[0])
@@ -92,8 +97,9 @@ def test_adjoint_matches_f64_oracle_and_finite_differences(gpu_ctx, n, seed):
             xm[k] -= h
             lp, lm, l0 = f(xp), f(xm), f(x0)
             c, r, l = (lp - lm) / (2 * h), (lp - l0) / h, (l0 - lm) / h
-            if abs(r - l) <= 1e-3 * max(abs(c), 1.0):  # no kink inside [x - h, x + h]
+            if abs(r - l) <= 2e-2 * max(abs(c), 1.0):
                 assert abs(gk - c) <= 1e-3 * max(abs(c), 1.0), (i, k, gk, c)
                 checked_fd += 1
+                break
     print(f"adjoint n={n}: worst gradient rel err vs f64 adjoint {worst_g:.2e}, fd checks {checked_fd}", flush=True)
     assert checked_fd >= 32
