@@ -33,167 +33,254 @@ struct PathSrc {
     const int64_t* weight_off;
 };
 
-// Output cursor: counts when verbs == nullptr.
-struct Emitter {
-    uint8_t* verbs;
-    float* points;
-    int64_t v = 0, q = 0;
-    __device__ void push(int verb, V2 p) {
-        if (verbs) {
-            verbs[v] = (uint8_t)verb;
-            if (verb != CLOSE) {
-                points[2 * q] = p.x;
-                points[2 * q + 1] = p.y;
+// ------------------------------------------------------------ flatten
+// flatten (path.rs:114-144), one wave per path, one lane per command (64 at
+// a time).  A lane's outputs: Move -> M, Line -> L, Close -> Z, a curve ->
+// its segments' end points as L (path.rs:49-104); `last` is the previous
+// command's end point, which is the point before the command's own.
+// Returns the error bits of the lane.
+template <bool EMIT>
+__global__ __launch_bounds__(WAVE) void k_sflatten(PathSrc in, int64_t n, int64_t* cnt, const int64_t* off_v,
+                                                  const int64_t* off_p, uint8_t* verbs, float* points,
+                                                  uint32_t* error) {
+    const int64_t p = blockIdx.x;
+    if (p >= n) return;
+    const int lane = threadIdx.x;
+    const int64_t vb = in.verb_off[p], ve = in.verb_off[p + 1];
+    const int64_t pb = in.point_off[p], pe = in.point_off[p + 1];
+    const int64_t wb = in.weight_off[p], we = in.weight_off[p + 1];
+    uint32_t cpts = 0, ccon = 0, err = 0;
+    uint64_t ov = EMIT ? (uint64_t)off_v[p] : 0u, op = EMIT ? (uint64_t)off_p[p] : 0u;
+    for (int64_t cb = vb; cb < ve; cb += WAVE) {
+        const int64_t i = cb + lane;
+        const bool valid = i < ve;
+        const int verb = valid ? in.verbs[i] : (int)CLOSE;
+        int k = verb_points(verb);
+        bool bad = valid && k < 0;
+        if (k < 0) k = 0;
+        const uint32_t k_incl = wave_incl_scan((uint32_t)k, OpAdd());
+        const uint32_t cc = (valid && verb == CONIC) ? 1u : 0u;
+        const uint32_t cc_incl = wave_incl_scan(cc, OpAdd());
+        const int64_t g = pb + cpts + (k_incl - (uint32_t)k);
+        const int64_t wi = wb + ccon + (cc_incl - cc);
+        if (valid && (g + k > pe || (cc && wi >= we))) bad = true;
+        V2 P0 = mk(0.0f, 0.0f), P1 = P0, P2 = P0, S = P0;
+        float w = 0.0f;
+        if (valid && !bad) {
+            if (k > 0) P0 = mk(in.points[2 * g], in.points[2 * g + 1]);
+            if (k > 1) P1 = mk(in.points[2 * g + 2], in.points[2 * g + 3]);
+            if (k > 2) P2 = mk(in.points[2 * g + 4], in.points[2 * g + 5]);
+            if (g > pb) S = mk(in.points[2 * g - 2], in.points[2 * g - 1]);
+            if (cc) w = in.weights[wi];
+        }
+        // outputs of this command: verbs (M/L/Z) and points
+        uint32_t nv = 0;
+        float dt = 0.0f;
+        if (valid && !bad) {
+            if (verb == MOVE || verb == LINE || verb == CLOSE) {
+                nv = 1;
+            } else if (verb == QUAD || verb == CUBIC) {
+                dt = verb == QUAD ? quad_dt(S, P0, P1) : cubic_dt(S, P0, P1, P2);
+                float t = 0.0f;
+                while (t < 1.0f) {
+                    t = fminf(t + dt, 1.0f);
+                    if (++nv >= FLATTEN_LIMIT) {
+                        err |= ERR_FLATTEN;
+                        break;
+                    }
+                }
+            } else {
+                if (!conic_walk(S, P0, P1, w, [&](V2) { nv++; })) err |= ERR_FLATTEN;
             }
         }
-        v++;
-        if (verb != CLOSE) q++;
+        if (bad) err |= ERR_INPUT;
+        const uint32_t npt = (valid && verb != CLOSE) ? nv : 0u;
+        const uint32_t v_incl = wave_incl_scan(nv, OpAdd());
+        const uint32_t p_incl = wave_incl_scan(npt, OpAdd());
+        if (EMIT && valid && !bad && !err) {
+            uint64_t vo = ov + (v_incl - nv), po = op + (p_incl - npt);
+            auto put = [&](int vv, V2 q) {
+                verbs[vo++] = (uint8_t)vv;
+                points[2 * po] = q.x;
+                points[2 * po + 1] = q.y;
+                po++;
+            };
+            if (verb == MOVE || verb == LINE) {
+                put(verb, P0);
+            } else if (verb == CLOSE) {
+                verbs[vo] = (uint8_t)CLOSE;
+            } else if (verb == QUAD || verb == CUBIC) {
+                float t = 0.0f;
+                while (t < 1.0f) {
+                    t = fminf(t + dt, 1.0f);
+                    put(LINE, verb == QUAD ? quad_at(t, S, P0, P1) : cubic_at(t, S, P0, P1, P2));
+                }
+            } else {
+                conic_walk(S, P0, P1, w, [&](V2 q) { put(LINE, q); });
+            }
+        }
+        ov += __shfl(v_incl, WAVE - 1, WAVE);
+        op += __shfl(p_incl, WAVE - 1, WAVE);
+        cpts += __shfl(k_incl, WAVE - 1, WAVE);
+        ccon += __shfl(cc_incl, WAVE - 1, WAVE);
     }
+    if (cpts != (uint32_t)(pe - pb)) err |= ERR_INPUT;
+    if (!EMIT && lane == 0) {
+        cnt[p] = (int64_t)ov;
+        cnt[n + p] = (int64_t)op;
+    }
+    if (err) atomicOr(error, err);
+}
+
+// ------------------------------------------------------------- stroke
+// stroke (path.rs:152-274), one wave per path.  The flattened commands are
+// cut into contours exactly where the reference processes them (a Move, a
+// Close, the end); a contour holds Move/Line commands only, so its points
+// are contiguous.  Each side of a contour (offset, path.rs:174-215) is a pass
+// over i = 0..n, 64 indices at a time: index i is a join event when its point
+// differs from the previous one (or i = n), the join uses the normal of the
+// previous event (a ballot finds it), and each join writes one point (miter)
+// or two (bevel) at positions from a wave scan.
+
+__device__ inline V2 unit_normal(V2 a, V2 b) {
+    const V2 tangent = sub(b, a);
+    const V2 normal = mk(-tangent.y, tangent.x);
+    const float len = vlen(normal);
+    return len == 0.0f ? mk(0.0f, 0.0f) : smul(1.0f / len, normal);
+}
+
+struct StrokeOut {
+    uint8_t* verbs;  // null: count only
+    float* points;
+    uint64_t v, q;   // cursors (uniform across the wave)
 };
 
-// flatten (path.rs:114-144).  Returns error bits: ERR_INPUT for an unknown
-// verb or a path whose points/weights do not match its verbs, ERR_FLATTEN
-// when a curve needs more than FLATTEN_LIMIT segments (the reference would
-// not terminate).
-__device__ uint32_t flatten_one(const PathSrc& in, int64_t p, Emitter& out) {
-    const int64_t vb = in.verb_off[p], ve = in.verb_off[p + 1];
-    int64_t g = in.point_off[p], wi = in.weight_off[p];
-    const int64_t ge = in.point_off[p + 1], we = in.weight_off[p + 1];
-    V2 last = mk(0.0f, 0.0f);
-    for (int64_t i = vb; i < ve; i++) {
-        const int verb = in.verbs[i];
-        const int k = verb_points(verb);
-        if (k < 0 || g + k > ge || (verb == CONIC && wi >= we)) return ERR_INPUT;
-        V2 pt[3] = {mk(0.0f, 0.0f), mk(0.0f, 0.0f), mk(0.0f, 0.0f)};  // static indices: registers
-        if (k > 0) pt[0] = mk(in.points[2 * g], in.points[2 * g + 1]);
-        if (k > 1) pt[1] = mk(in.points[2 * (g + 1)], in.points[2 * (g + 1) + 1]);
-        if (k > 2) pt[2] = mk(in.points[2 * (g + 2)], in.points[2 * (g + 2) + 1]);
-        if (verb == MOVE || verb == LINE) {
-            out.push(verb, pt[0]);
-        } else if (verb == QUAD || verb == CUBIC) {
-            const float dt = verb == QUAD ? quad_dt(last, pt[0], pt[1]) : cubic_dt(last, pt[0], pt[1], pt[2]);
-            float t = 0.0f;
-            uint32_t n = 0;
-            while (t < 1.0f) {
-                t = fminf(t + dt, 1.0f);
-                out.push(LINE, verb == QUAD ? quad_at(t, last, pt[0], pt[1]) : cubic_at(t, last, pt[0], pt[1], pt[2]));
-                if (++n >= FLATTEN_LIMIT) return ERR_FLATTEN;
-            }
-        } else if (verb == CONIC) {
-            if (!conic_walk(last, pt[0], pt[1], in.weights[wi], [&](V2 r) { out.push(LINE, r); })) return ERR_FLATTEN;
-            wi++;
-        } else {
-            out.push(CLOSE, mk(0.0f, 0.0f));
-        }
-        if (k == 1) last = pt[0];
-        else if (k == 2) last = pt[1];
-        else if (k == 3) last = pt[2];
-        g += k;
-    }
-    return g == ge && wi == we ? 0u : (uint32_t)ERR_INPUT;
-}
-
-// join (path.rs:163-171)
-template <class Sink>
-__device__ inline void stroke_join(float width, V2 prev_n, V2 next_n, V2 point, Sink&& sink) {
-    const float offset = 1.0f / (1.0f + dot(prev_n, next_n));
-    if (fabsf(offset) > 2.0f) {
-        sink(add(point, smul(0.5f * width, prev_n)));
-        sink(add(point, smul(0.5f * width, next_n)));
-    } else {
-        sink(add(point, smul(0.5f * width * offset, add(prev_n, next_n))));
-    }
-}
-
-// offset (path.rs:174-215) over a contour of n >= 1 points c[0..n).
-template <class Sink>
-__device__ inline void stroke_offset(const float* c, int64_t n, bool closed, bool reverse, float width, Sink&& sink) {
-    auto at = [&](int64_t k) { return mk(c[2 * k], c[2 * k + 1]); };
+// One side of one contour of n >= 1 points at pts[0 .. n) (path.rs:174-215).
+// `first_move`: the side's first output is a Move (else a Line).
+template <bool EMIT>
+__device__ void stroke_side(const float* pts, uint32_t n, bool closed, bool reverse, float width, bool first_move,
+                            StrokeOut& o) {
+    const int lane = threadIdx.x;
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    auto at = [&](uint32_t k) { return mk(pts[2 * k], pts[2 * k + 1]); };
     const V2 first_point = closed == reverse ? at(0) : at(n - 1);
-    V2 prev_point = first_point;
-    V2 prev_normal = mk(0.0f, 0.0f);
-    for (int64_t i = 0; i <= n; i++) {
-        const V2 next_point = i < n ? at(reverse ? n - i - 1 : i) : first_point;
-        if (!veq(next_point, prev_point) || i == n) {
-            const V2 tangent = sub(next_point, prev_point);
-            const V2 normal = mk(-tangent.y, tangent.x);
-            const float len = vlen(normal);
-            const V2 unit = len == 0.0f ? mk(0.0f, 0.0f) : smul(1.0f / len, normal);
-            stroke_join(width, prev_normal, unit, prev_point, sink);
-            prev_point = next_point;
-            prev_normal = unit;
+    V2 carry_pt = first_point;            // point of index i0 - 1 (for i0 = 0: first_point)
+    V2 carry_n = mk(0.0f, 0.0f);          // normal of the last event so far
+    bool first_out = true;
+    for (uint32_t i0 = 0; i0 <= n; i0 += WAVE) {
+        const uint32_t i = i0 + (uint32_t)lane;
+        const bool in_range = i <= n;
+        V2 pt = first_point;
+        if (i < n) pt = at(reverse ? n - 1 - i : i);
+        // previous point: lane - 1, or the carry for lane 0
+        V2 prv = mk(__shfl_up(pt.x, 1, WAVE), __shfl_up(pt.y, 1, WAVE));
+        if (lane == 0) prv = carry_pt;
+        const bool event = in_range && (!veq(pt, prv) || i == n);
+        const V2 nrm = event ? unit_normal(prv, pt) : mk(0.0f, 0.0f);
+        // the previous event's normal
+        const uint64_t ev = __ballot(event);
+        const uint64_t before = ev & lt;
+        const int src = before ? 63 - __clzll((long long)before) : 0;
+        V2 pn = mk(__shfl(nrm.x, src, WAVE), __shfl(nrm.y, src, WAVE));
+        if (!before) pn = carry_n;
+        // join (path.rs:163-171) at the previous point
+        V2 j0 = mk(0.0f, 0.0f), j1 = j0;
+        uint32_t cnt = 0;
+        if (event) {
+            const float offset = 1.0f / (1.0f + dot(pn, nrm));
+            if (fabsf(offset) > 2.0f) {
+                j0 = add(prv, smul(0.5f * width, pn));
+                j1 = add(prv, smul(0.5f * width, nrm));
+                cnt = 2;
+            } else {
+                j0 = add(prv, smul(0.5f * width * offset, add(pn, nrm)));
+                cnt = 1;
+            }
+        }
+        const uint32_t c_incl = wave_incl_scan(cnt, OpAdd());
+        const uint32_t total = __shfl(c_incl, WAVE - 1, WAVE);
+        if (EMIT && cnt) {
+            const uint64_t pos = o.q + (c_incl - cnt);
+            const bool is_first = first_out && pos == o.q;
+            o.verbs[o.v + (c_incl - cnt)] = (uint8_t)(is_first && first_move ? MOVE : LINE);
+            o.points[2 * pos] = j0.x;
+            o.points[2 * pos + 1] = j0.y;
+            if (cnt == 2) {
+                o.verbs[o.v + (c_incl - cnt) + 1] = (uint8_t)LINE;
+                o.points[2 * pos + 2] = j1.x;
+                o.points[2 * pos + 3] = j1.y;
+            }
+        }
+        o.v += total;
+        o.q += total;
+        if (total) first_out = false;
+        // carries for the next 64 indices
+        const int lastl = WAVE - 1;
+        carry_pt = mk(__shfl(pt.x, lastl, WAVE), __shfl(pt.y, lastl, WAVE));
+        if (ev) {
+            const int le = 63 - __clzll((long long)ev);
+            carry_n = mk(__shfl(nrm.x, le, WAVE), __shfl(nrm.y, le, WAVE));
         }
     }
 }
 
-// stroke (path.rs:152-274) of one flattened path.  A contour holds only Move
-// and Line commands (a Close ends it and is skipped), so its points are
-// contiguous: command cs of the polyline is point qs, and the contour
-// [cs, ce) is points [qs, qs + ce - cs).  `closed` is never cleared once a
-// Close has been seen, as in the reference.
-__device__ void stroke_one(const uint8_t* verbs, const float* pts, int64_t nv, float width, Emitter& out) {
-    int64_t cs = 0, ce = 0, qs = 0, q = 0;
+template <bool EMIT>
+__device__ inline void put_close(StrokeOut& o) {
+    if (EMIT && threadIdx.x == 0) o.verbs[o.v] = (uint8_t)CLOSE;
+    o.v += 1;
+}
+
+template <bool EMIT>
+__global__ __launch_bounds__(WAVE) void k_sstroke(const uint8_t* fverbs, const float* fpoints, const int64_t* foff_v,
+                                                 const int64_t* foff_p, const float* widths, int64_t n, int64_t* cnt,
+                                                 const int64_t* off_v, const int64_t* off_p, uint8_t* verbs,
+                                                 float* points) {
+    const int64_t p = blockIdx.x;
+    if (p >= n) return;
+    const int lane = threadIdx.x;
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    const uint8_t* v = fverbs + foff_v[p];
+    const float* pts = fpoints + 2 * foff_p[p];
+    const uint32_t nv = (uint32_t)(foff_v[p + 1] - foff_v[p]);
+    const float width = widths[p];
+    StrokeOut o{EMIT ? verbs + off_v[p] : nullptr, EMIT ? points + 2 * off_p[p] : nullptr, 0, 0};
+    // contour [cs, b) of point-bearing commands starting at point qs
+    uint32_t cs = 0, qs = 0, closes = 0;
     bool closed = false;
-    for (int64_t i = 0;; i++) {
-        const int verb = i < nv ? verbs[i] : -1;
-        if (verb == CLOSE) closed = true;
-        if ((verb < 0 || verb == MOVE || verb == CLOSE) && cs != ce) {
-            const float* c = pts + 2 * qs;
-            const int64_t n = ce - cs;
-            bool first = true;
-            stroke_offset(c, n, closed, false, width, [&](V2 r) {
-                out.push(first ? MOVE : LINE, r);
-                first = false;
-            });
-            if (closed) out.push(CLOSE, mk(0.0f, 0.0f));
-            first = true;
-            stroke_offset(c, n, closed, true, width, [&](V2 r) {
-                out.push(first && closed ? MOVE : LINE, r);
-                first = false;
-            });
-            out.push(CLOSE, mk(0.0f, 0.0f));
+    auto contour = [&](uint32_t b) {
+        if (b <= cs) return;
+        const uint32_t m = b - cs;
+        const float* c = pts + 2 * qs;
+        stroke_side<EMIT>(c, m, closed, false, width, true, o);
+        if (closed) put_close<EMIT>(o);
+        stroke_side<EMIT>(c, m, closed, true, width, closed, o);
+        put_close<EMIT>(o);
+    };
+    for (uint32_t i0 = 0; i0 < nv; i0 += WAVE) {
+        const uint32_t i = i0 + (uint32_t)lane;
+        const int verb = i < nv ? v[i] : -1;
+        const uint64_t bmask = __ballot(verb == MOVE || verb == CLOSE);
+        const uint64_t cmask = __ballot(verb == CLOSE);
+        uint64_t m = bmask;
+        while (m) {
+            const int l = __ffsll((long long)m) - 1;
+            m &= m - 1;
+            const uint32_t b = i0 + (uint32_t)l;
+            const bool is_close = (cmask >> l) & 1ull;
+            const uint32_t closes_before = closes + (uint32_t)__popcll(cmask & ((1ull << l) - 1ull));
+            if (is_close) closed = true;
+            contour(b);
+            // the next contour starts at the Move, or after the Close
+            cs = is_close ? b + 1 : b;
+            qs = cs - (closes_before + (is_close ? 1u : 0u));
         }
-        if (verb < 0) break;
-        if (verb == MOVE) {
-            cs = ce;
-            ce = cs + 1;
-            qs = q;
-        } else if (verb == LINE) {
-            ce += 1;
-        } else {
-            cs = ce + 1;
-            ce = cs;
-            closed = true;
-            qs = q;
-        }
-        if (verb != CLOSE) q++;
+        closes += (uint32_t)__popcll(cmask);
     }
-}
-
-// Per-path sizes: cnt[0 .. n) verbs and cnt[n .. 2n) points.
-__global__ void k_flatten_paths(PathSrc in, int64_t n, int64_t* cnt, const int64_t* off_v, const int64_t* off_p,
-                                uint8_t* verbs, float* points, uint32_t* error) {
-    const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    Emitter e{verbs ? verbs + off_v[p] : nullptr, verbs ? points + 2 * off_p[p] : nullptr};
-    const uint32_t err = flatten_one(in, p, e);
-    if (err) atomicOr(error, err);
-    if (!verbs) {
-        cnt[p] = e.v;
-        cnt[n + p] = e.q;
-    }
-}
-
-__global__ void k_stroke_paths(const uint8_t* fverbs, const float* fpoints, const int64_t* foff_v,
-                               const int64_t* foff_p, const float* widths, int64_t n, int64_t* cnt,
-                               const int64_t* off_v, const int64_t* off_p, uint8_t* verbs, float* points) {
-    const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    Emitter e{verbs ? verbs + off_v[p] : nullptr, verbs ? points + 2 * off_p[p] : nullptr};
-    stroke_one(fverbs + foff_v[p], fpoints + 2 * foff_p[p], foff_v[p + 1] - foff_v[p], widths[p], e);
-    if (!verbs) {
-        cnt[p] = e.v;
-        cnt[n + p] = e.q;
+    contour(nv);
+    (void)lt;
+    if (!EMIT && lane == 0) {
+        cnt[p] = (int64_t)o.v;
+        cnt[n + p] = (int64_t)o.q;
     }
 }
 
@@ -251,9 +338,9 @@ int stroke_expand(Context* ctx, const SegInput& in, const float* widths, StrokeO
     PathSrc src{in.verbs, in.seg_verb_off, in.points, in.seg_point_off, in.weights, in.seg_weight_off};
     int64_t* cnt = B.cnt.as<int64_t>();
     uint32_t* err = B.err.as<uint32_t>();
-    const uint32_t blocks = (uint32_t)((n + 63) / 64);
+    const uint32_t blocks = (uint32_t)n;  // one wave per path
     hipLaunchKernelGGL(k_zero_i64, dim3(1), dim3(64), 0, s, B.err.as<int64_t>(), (int64_t)2);
-    hipLaunchKernelGGL(k_flatten_paths, dim3(blocks), dim3(64), 0, s, src, n, cnt, nullptr, nullptr, nullptr,
+    hipLaunchKernelGGL(k_sflatten<false>, dim3(blocks), dim3(WAVE), 0, s, src, n, cnt, nullptr, nullptr, nullptr,
                        nullptr, err);
     HIPCHK_RET(offsets_i64(ctx, cnt, B.fv.as<int64_t>(), n, s));
     HIPCHK_RET(offsets_i64(ctx, cnt + n, B.fp.as<int64_t>(), n, s));
@@ -266,10 +353,11 @@ int stroke_expand(Context* ctx, const SegInput& in, const float* widths, StrokeO
     const int64_t FV = h[0], FP = h[1];
     HIPCHK_RET(B.fverbs.ensure((size_t)std::max<int64_t>(FV, 1)));
     HIPCHK_RET(B.fpoints.ensure((size_t)std::max<int64_t>(FP, 1) * 8));
-    hipLaunchKernelGGL(k_flatten_paths, dim3(blocks), dim3(64), 0, s, src, n, cnt, B.fv.as<int64_t>(),
+    hipLaunchKernelGGL(k_sflatten<true>, dim3(blocks), dim3(WAVE), 0, s, src, n, cnt, B.fv.as<int64_t>(),
                        B.fp.as<int64_t>(), B.fverbs.as<uint8_t>(), B.fpoints.as<float>(), err);
-    hipLaunchKernelGGL(k_stroke_paths, dim3(blocks), dim3(64), 0, s, B.fverbs.as<uint8_t>(), B.fpoints.as<float>(),
-                       B.fv.as<int64_t>(), B.fp.as<int64_t>(), widths, n, cnt, nullptr, nullptr, nullptr, nullptr);
+    hipLaunchKernelGGL(k_sstroke<false>, dim3(blocks), dim3(WAVE), 0, s, B.fverbs.as<uint8_t>(),
+                       B.fpoints.as<float>(), B.fv.as<int64_t>(), B.fp.as<int64_t>(), widths, n, cnt, nullptr,
+                       nullptr, nullptr, nullptr);
     HIPCHK_RET(offsets_i64(ctx, cnt, B.ov.as<int64_t>(), n, s));
     HIPCHK_RET(offsets_i64(ctx, cnt + n, B.op.as<int64_t>(), n, s));
     HIPCHK_RET(hipMemcpyAsync(h, B.ov.as<int64_t>() + n, 8, hipMemcpyDeviceToHost, s));
@@ -278,9 +366,9 @@ int stroke_expand(Context* ctx, const SegInput& in, const float* widths, StrokeO
     const int64_t OV = h[0], OP = h[1];
     HIPCHK_RET(B.overbs.ensure((size_t)std::max<int64_t>(OV, 1)));
     HIPCHK_RET(B.opoints.ensure((size_t)std::max<int64_t>(OP, 1) * 8));
-    hipLaunchKernelGGL(k_stroke_paths, dim3(blocks), dim3(64), 0, s, B.fverbs.as<uint8_t>(), B.fpoints.as<float>(),
-                       B.fv.as<int64_t>(), B.fp.as<int64_t>(), widths, n, cnt, B.ov.as<int64_t>(),
-                       B.op.as<int64_t>(), B.overbs.as<uint8_t>(), B.opoints.as<float>());
+    hipLaunchKernelGGL(k_sstroke<true>, dim3(blocks), dim3(WAVE), 0, s, B.fverbs.as<uint8_t>(),
+                       B.fpoints.as<float>(), B.fv.as<int64_t>(), B.fp.as<int64_t>(), widths, n, cnt,
+                       B.ov.as<int64_t>(), B.op.as<int64_t>(), B.overbs.as<uint8_t>(), B.opoints.as<float>());
     // outlines hold no conics: every weight offset is zero
     hipLaunchKernelGGL(k_zero_i64, dim3((uint32_t)std::min<int64_t>((n + 256) / 256, 1024)), dim3(256), 0, s,
                        B.ow.as<int64_t>(), n + 1);
