@@ -222,6 +222,12 @@ int ochre_context_get_batched_params(ochre_context* ctx, int* waves_per_cu, int*
  * waves_per_group 2 or 4 (waves sharing one path). */
 int ochre_context_set_dense_params(ochre_context* ctx, int enabled, int groups_per_cu, int waves_per_group);
 int ochre_context_get_dense_params(ochre_context* ctx, int* enabled, int* groups_per_cu, int* waves_per_group);
+/* A small-grid dense instance (tile grids of at most 32 tiles, 2 waves and
+ * 16 KB of LDS per workgroup, so twice as many paths in flight per CU) runs in
+ * front of the 64-tile one and hands it the larger paths; groups_per_cu 0
+ * (off) to 9, default 8. */
+int ochre_context_set_dense_small(ochre_context* ctx, int groups_per_cu);
+int ochre_context_get_dense_small(ochre_context* ctx, int* groups_per_cu);
 
 /* Coverage adjoint.  The loss of path p is the sum, over the pixels of the
  * union of its tiles and its target's tiles, of (cov - target)^2, where a
@@ -247,8 +253,9 @@ int ochre_coverage_adjoint(ochre_context* ctx, const ochre_path_batch* in, const
                            float* loss, float* d_points, float* d_transforms, void* stream);
 
 /* Autotuner: with the dense kernel on, times every (workgroups per CU
- * {1,2,3,4}, waves per workgroup {2,4}) of the dense launch, then waves per
- * CU {1,2,4,8} of the ordered kernel behind it; with it off,
+ * {1,2,3,4}, waves per workgroup {2,4}) of the 64-tile dense launch, then
+ * the small-grid instance {off, 4, 6, 8 workgroups per CU}, then waves per
+ * CU {1,2,4,8} of the ordered kernel behind them; with it off,
  * every combination of waves per CU {2,3,4,6,8,12}, paths per pop {1,2,4,8}
  * and tiles per chunk {16,32} of the ordered launch.  Each configuration is
  * timed on the caller's batch (`reps` calls, median); the fastest is kept in
@@ -261,6 +268,7 @@ typedef struct ochre_batched_tuning {
     float ms;
     int dense_groups_per_cu; /* 0: the dense kernel was off */
     int dense_waves_per_group;
+    int dense_small_groups_per_cu; /* the small-grid dense instance (0: off) */
 } ochre_batched_tuning;
 int ochre_autotune_batched(ochre_context* ctx, const ochre_path_batch* in, ochre_batch_output* out, void* stream,
                            int reps, ochre_batched_tuning* results, int max_results, int* n_results);

@@ -29,6 +29,7 @@ EXPORTS = [
     "ochre_context_get_batched_params", "ochre_autotune_batched", "ochre_coverage_adjoint",
     "ochre_rasterizer_stroke", "ochre_stroke_plain", "ochre_stroke_batched",
     "ochre_context_set_dense_params", "ochre_context_get_dense_params", "ochre_build_atlas",
+    "ochre_context_set_dense_small", "ochre_context_get_dense_small",
 ]
 
 
@@ -70,7 +71,7 @@ class CoverageTargetC(ctypes.Structure):
 class BatchedTuningC(ctypes.Structure):
     _fields_ = [("waves_per_cu", ctypes.c_int), ("paths_per_pop", ctypes.c_int), ("tiles_per_chunk", ctypes.c_int),
                 ("ms", ctypes.c_float), ("dense_groups_per_cu", ctypes.c_int),
-                ("dense_waves_per_group", ctypes.c_int)]
+                ("dense_waves_per_group", ctypes.c_int), ("dense_small_groups_per_cu", ctypes.c_int)]
 
 
 TILE_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int16, ctypes.c_int16, ctypes.POINTER(ctypes.c_uint8))
@@ -127,6 +128,8 @@ def _optional(lib):
         I = ctypes.c_int
         lib.ochre_context_set_dense_params.argtypes = [P, I, I, I]
         lib.ochre_context_get_dense_params.argtypes = [P] + [ctypes.POINTER(I)] * 3
+        lib.ochre_context_set_dense_small.argtypes = [P, I]
+        lib.ochre_context_get_dense_small.argtypes = [P, ctypes.POINTER(I)]
     if hasattr(lib, "ochre_build_atlas"):
         i64 = ctypes.c_int64
         lib.ochre_build_atlas.argtypes = [P, ctypes.POINTER(BatchOutputC), i64, P, P, ctypes.c_int, P, i64, P, i64,

@@ -79,7 +79,9 @@ struct FusedCtl {
     unsigned long long tiles;
     unsigned long long spans;
     uint32_t queue2;    // queue of k_fused over the handed-on paths
-    uint32_t pad[55];
+    uint32_t queue3;    // queue of the large-grid dense kernel
+    uint32_t dq2;       // paths handed on by the large-grid dense kernel
+    uint32_t pad[53];
 };
 
 __global__ void k_fused_reset(FusedCtl* c) {
@@ -91,6 +93,8 @@ __global__ void k_fused_reset(FusedCtl* c) {
         c->tiles = 0;
         c->spans = 0;
         c->queue2 = 0;
+        c->queue3 = 0;
+        c->dq2 = 0;
     }
 }
 
@@ -784,7 +788,7 @@ __global__ void k_scatter_ranges(const uint32_t* ids, uint32_t n, const int64_t*
 }
 
 struct BatchedState {
-    DBuf scratch, ascratch, ctl, deferred, dscratch, dense_ids;
+    DBuf scratch, ascratch, ctl, deferred, dscratch, dense_ids, dscratch2, dense_ids2;
     DBuf g_verbs, g_points, g_weights, g_voff, g_poff, g_woff, g_xf, g_ts, g_tc, g_ss, g_sc;
     int n_waves = 0;
     uint32_t gran = 1;
@@ -845,42 +849,60 @@ int run_batched(Context* ctx, const SegInput& in, OutputDesc& out, hipStream_t s
     const bool dense = ctx->dense;
     const int dnw = ctx->dense_waves == 2 ? 2 : 4;
     const int n_groups = std::max(1, cus * std::max(1, ctx->dense_groups_per_cu));
+    const int n_small = dense ? cus * std::max(0, ctx->dense_small_groups) : 0;
     uint8_t* dscratch = nullptr;
-    uint32_t* dense_ids = nullptr;
-    FusedIn fi2 = fi;
+    uint8_t* dscratch2 = nullptr;
+    uint32_t* ids_a = nullptr;
+    uint32_t* ids_b = nullptr;
+    FusedIn fi_a = fi, fi_b = fi;
     if (dense) {
         HIPCHK_RET(B->dscratch.ensure((size_t)n_groups * D_SCRATCH));
         HIPCHK_RET(B->dense_ids.ensure((size_t)NP * 4));
+        HIPCHK_RET(B->dense_ids2.ensure((size_t)NP * 4));
+        if (n_small) HIPCHK_RET(B->dscratch2.ensure((size_t)n_small * D_SCRATCH));
         dscratch = B->dscratch.as<uint8_t>();
-        dense_ids = B->dense_ids.as<uint32_t>();
-        fi2.ids = dense_ids;
-        fi2.n_ids = &ctl->dq;
+        dscratch2 = B->dscratch2.as<uint8_t>();
+        ids_a = B->dense_ids.as<uint32_t>();
+        ids_b = B->dense_ids2.as<uint32_t>();
+        // small grids first (all paths -> list a), then large grids (list a ->
+        // list b), then the ordered kernel (list b)
+        fi_a.ids = ids_a;
+        fi_a.n_ids = &ctl->dq;
+        fi_b.ids = ids_b;
+        fi_b.n_ids = &ctl->dq2;
     }
     const bool timing = ctx->timing;
     if (timing) {
         for (auto& e : ctx->ev)
             if (!e) HIPCHK_RET(hipEventCreate(&e));
     }
-    // timing without a graph brackets each kernel with events
+    // timing without a graph brackets the dense kernels and k_fused with events
     auto enqueue = [&](hipStream_t st, bool ev) {
         hipLaunchKernelGGL(k_fused_reset, dim3(1), dim3(64), 0, st, ctl);
         if (ev) (void)hipEventRecord(ctx->ev[0], st);
         if (dense) {
+            const FusedIn& f_large = n_small ? fi_a : fi;
+            if (n_small)
+                hipLaunchKernelGGL((k_dense<2, 32>), dim3(n_small), dim3(2 * WAVE), 0, st, fi, fo, ctl, &ctl->queue,
+                                   ids_a, &ctl->dq, dscratch2);
+            uint32_t* out_ids = n_small ? ids_b : ids_a;
+            uint32_t* out_cnt = n_small ? &ctl->dq2 : &ctl->dq;
+            uint32_t* q = n_small ? &ctl->queue3 : &ctl->queue;
             if (dnw == 2)
-                hipLaunchKernelGGL(k_dense<2>, dim3(n_groups), dim3(2 * WAVE), 0, st, fi, fo, ctl, dense_ids,
-                                   dscratch);
+                hipLaunchKernelGGL((k_dense<2, 64>), dim3(n_groups), dim3(2 * WAVE), 0, st, f_large, fo, ctl, q,
+                                   out_ids, out_cnt, dscratch);
             else
-                hipLaunchKernelGGL(k_dense<4>, dim3(n_groups), dim3(4 * WAVE), 0, st, fi, fo, ctl, dense_ids,
-                                   dscratch);
+                hipLaunchKernelGGL((k_dense<4, 64>), dim3(n_groups), dim3(4 * WAVE), 0, st, f_large, fo, ctl, q,
+                                   out_ids, out_cnt, dscratch);
             if (ev) (void)hipEventRecord(ctx->ev[2], st);
         }
-        const FusedIn& f = dense ? fi2 : fi;
+        const FusedIn& f = dense ? (n_small ? fi_b : fi_a) : fi;
         if (tc16) hipLaunchKernelGGL(k_fused<16>, dim3(n_waves), dim3(WAVE), 0, st, f, fo, ctl, deferred, scratch, g);
         else hipLaunchKernelGGL(k_fused<32>, dim3(n_waves), dim3(WAVE), 0, st, f, fo, ctl, deferred, scratch, g);
         if (ev) (void)hipEventRecord(ctx->ev[1], st);
     };
     if (use_graph) {
-        std::vector<uint8_t> key(sizeof(fi) + sizeof(fo) + 33);
+        std::vector<uint8_t> key(sizeof(fi) + sizeof(fo) + 41);
         key.back() = tc16 ? 1 : 0;
         memcpy(key.data(), &fi, sizeof(fi));
         memcpy(key.data() + sizeof(fi), &fo, sizeof(fo));
@@ -890,6 +912,7 @@ int run_batched(Context* ctx, const SegInput& in, OutputDesc& out, hipStream_t s
         memcpy(key.data() + sizeof(fi) + sizeof(fo) + 16, &dscratch, 8);
         const int dkey = dense ? (dnw << 16) | n_groups : 0;
         memcpy(key.data() + sizeof(fi) + sizeof(fo) + 24, &dkey, 4);
+        memcpy(key.data() + sizeof(fi) + sizeof(fo) + 28, &n_small, 4);
         if (!B->exec || key != B->graph_key) {
             if (B->exec) (void)hipGraphExecDestroy(B->exec);
             if (B->graph) (void)hipGraphDestroy(B->graph);

@@ -257,6 +257,20 @@ int ochre_context_set_dense_params(ochre_context* ctx, int enabled, int groups_p
     return OCHRE_OK;
 }
 
+int ochre_context_set_dense_small(ochre_context* ctx, int groups_per_cu) {
+    if (!ctx || groups_per_cu < 0 || groups_per_cu > 9) return OCHRE_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->c.dense_small_groups = groups_per_cu;
+    return OCHRE_OK;
+}
+
+int ochre_context_get_dense_small(ochre_context* ctx, int* groups_per_cu) {
+    if (!ctx || !groups_per_cu) return OCHRE_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    *groups_per_cu = ctx->c.dense_small_groups;
+    return OCHRE_OK;
+}
+
 int ochre_context_get_dense_params(ochre_context* ctx, int* enabled, int* groups_per_cu, int* waves_per_group) {
     if (!ctx) return OCHRE_ERR_INVALID_ARG;
     std::lock_guard<std::mutex> lk(ctx->mu);
@@ -303,7 +317,7 @@ int ochre_autotune_batched(ochre_context* ctx, const ochre_path_batch* in, ochre
     static const int tcs[] = {16, 32};
     static const int dgroups[] = {1, 2, 3, 4};
     static const int dwaves[] = {2, 4};
-    ochre_batched_tuning best{ctx->c.batched_waves_per_cu, ctx->c.batched_gran, ctx->c.batched_tc, 1e30f, 0, 0};
+    ochre_batched_tuning best{ctx->c.batched_waves_per_cu, ctx->c.batched_gran, ctx->c.batched_tc, 1e30f, 0, 0, 0};
     int best_dg = ctx->c.dense_groups_per_cu, best_dw = ctx->c.dense_waves;
     int k = 0;
     auto measure = [&](int w, int g, int tc, float* ms_out) -> int {
@@ -335,7 +349,7 @@ int ochre_autotune_batched(ochre_context* ctx, const ochre_path_batch* in, ochre
                 if (st) return st;
                 if (results && k < max_results)
                     results[k] = ochre_batched_tuning{best.waves_per_cu, best.paths_per_pop, best.tiles_per_chunk, ms,
-                                                      dg, dw};
+                                                      dg, dw, ctx->c.dense_small_groups};
                 k++;
                 if (ms < best.ms) {
                     best.ms = ms;
@@ -345,14 +359,32 @@ int ochre_autotune_batched(ochre_context* ctx, const ochre_path_batch* in, ochre
             }
         ctx->c.dense_groups_per_cu = best_dg;
         ctx->c.dense_waves = best_dw;
-        // then the persistent grid of the ordered kernel behind it, which
+        // the small-grid instance in front: off or 4-8 workgroups per CU
+        int best_small = ctx->c.dense_small_groups;
+        for (int sg : {0, 4, 6, 8}) {
+            ctx->c.dense_small_groups = sg;
+            float ms = 0;
+            st = measure(best.waves_per_cu, best.paths_per_pop, best.tiles_per_chunk, &ms);
+            if (st) return st;
+            if (results && k < max_results)
+                results[k] = ochre_batched_tuning{best.waves_per_cu, best.paths_per_pop, best.tiles_per_chunk, ms,
+                                                  best_dg, best_dw, sg};
+            k++;
+            if (ms < best.ms) {
+                best.ms = ms;
+                best_small = sg;
+            }
+        }
+        ctx->c.dense_small_groups = best_small;
+        // then the persistent grid of the ordered kernel behind them, which
         // only works through the paths the dense kernel hands on
         for (int w : {1, 2, 4, 8}) {
             float ms = 0;
             st = measure(w, best.paths_per_pop, best.tiles_per_chunk, &ms);
             if (st) return st;
             if (results && k < max_results)
-                results[k] = ochre_batched_tuning{w, best.paths_per_pop, best.tiles_per_chunk, ms, best_dg, best_dw};
+                results[k] = ochre_batched_tuning{w, best.paths_per_pop, best.tiles_per_chunk, ms, best_dg, best_dw,
+                                                  ctx->c.dense_small_groups};
             k++;
             if (ms < best.ms) {
                 best.ms = ms;
@@ -366,9 +398,9 @@ int ochre_autotune_batched(ochre_context* ctx, const ochre_path_batch* in, ochre
                     float ms = 0;
                     st = measure(w, g, tc, &ms);
                     if (st) return st;
-                    if (results && k < max_results) results[k] = ochre_batched_tuning{w, g, tc, ms, 0, 0};
+                    if (results && k < max_results) results[k] = ochre_batched_tuning{w, g, tc, ms, 0, 0, 0};
                     k++;
-                    if (ms < best.ms) best = ochre_batched_tuning{w, g, tc, ms, 0, 0};
+                    if (ms < best.ms) best = ochre_batched_tuning{w, g, tc, ms, 0, 0, 0};
                 }
     }
     if (n_results) *n_results = k;

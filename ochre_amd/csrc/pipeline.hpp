@@ -131,6 +131,7 @@ struct Context {
     bool dense = true;
     int dense_groups_per_cu = 4;
     int dense_waves = 4;
+    int dense_small_groups = 8;  // small-grid instance in front (0: off)
     StrokeBuffers stroke;
     AtlasState atlas;
     ~Context();

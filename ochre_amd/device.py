@@ -268,7 +268,15 @@ def dense_params(ctx: Context):
     en, gpc, wpg = (ctypes.c_int() for _ in range(3))
     _lib.check(ctx.lib.ochre_context_get_dense_params(ctx.h, ctypes.byref(en), ctypes.byref(gpc), ctypes.byref(wpg)),
                "get_dense_params")
-    return {"dense": en.value, "dense_groups_per_cu": gpc.value, "dense_waves_per_group": wpg.value}
+    sg = ctypes.c_int()
+    _lib.check(ctx.lib.ochre_context_get_dense_small(ctx.h, ctypes.byref(sg)), "get_dense_small")
+    return {"dense": en.value, "dense_groups_per_cu": gpc.value, "dense_waves_per_group": wpg.value,
+            "dense_small_groups_per_cu": sg.value}
+
+
+def set_dense_small(ctx: Context, groups_per_cu=8):
+    """The small-grid dense instance in front of the 64-tile one (0: off)."""
+    _lib.check(ctx.lib.ochre_context_set_dense_small(ctx.h, groups_per_cu), "set_dense_small")
 
 
 def set_dense_params(ctx: Context, enabled=True, groups_per_cu=4, waves_per_group=4):
@@ -300,6 +308,7 @@ def autotune(ctx: Context, db: DeviceBatch, out: DeviceOutput, reps=3, cache_key
             if "dense" in cached:
                 set_dense_params(ctx, bool(cached["dense"]), cached["dense_groups_per_cu"],
                                  cached["dense_waves_per_group"])
+                set_dense_small(ctx, cached.get("dense_small_groups_per_cu", 8))
             return cached, []
     res = (_lib.BatchedTuningC * 64)()
     n = ctypes.c_int()
@@ -308,6 +317,7 @@ def autotune(ctx: Context, db: DeviceBatch, out: DeviceOutput, reps=3, cache_key
                                               res, 64, ctypes.byref(n)), "ochre_autotune_batched")
     allm = [{"waves_per_cu": r.waves_per_cu, "paths_per_pop": r.paths_per_pop, "tiles_per_chunk": r.tiles_per_chunk,
              "dense_groups_per_cu": r.dense_groups_per_cu, "dense_waves_per_group": r.dense_waves_per_group,
+             "dense_small_groups_per_cu": r.dense_small_groups_per_cu,
              "ms": r.ms} for r in res[:n.value]]
     best = dict(batched_params(ctx))
     best["ms"] = min(m["ms"] for m in allm) if allm else None
