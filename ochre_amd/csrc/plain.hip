@@ -694,7 +694,26 @@ __global__ __launch_bounds__(WAVE) void k_tiles(TileArgs a) {
                 const uint32_t last_g = min(c + (uint32_t)TG, G1);
                 const uint32_t E0 = a.gbegin[c], E1 = a.gbegin[last_g];
                 const uint32_t my_e0 = hr ? a.gbegin[g] : 0u, my_e1 = hr ? a.gbegin[g + 1] : 0u;
-                for (uint32_t s0 = E0; s0 < E1; s0 += STAGE) {
+                // Light chunks stage their run records through LDS together
+                // (coalesced).  Heavy ones (large paths: hundreds of runs per
+                // tile, so one staging block would cover a single tile and the
+                // tiles would be replayed one after another) let every lane
+                // stream its own tile's runs from global memory instead.
+                const bool staged = E1 - E0 <= 2u * (uint32_t)STAGE;
+                if (!staged && hr) {
+                    uint32_t e = my_e0;
+                    uint32_t vn = e < my_e1 ? a.vals[e] : 0xffffffffu;
+                    while (e < my_e1) {
+                        const uint32_t v = vn;
+                        e++;
+                        vn = e < my_e1 ? a.vals[e] : 0xffffffffu;
+                        if (v >> 31) continue;  // a tile-row crossing, not a run
+                        const RunRec r = a.recs[v];
+                        const float4 L = a.lines[r.line & 0x7fffffffu];
+                        replay_run(r, L, (int16_t)tx, (int16_t)ty, acc_a, acc_h, lane, a.cnt);
+                    }
+                }
+                for (uint32_t s0 = E0; staged && s0 < E1; s0 += STAGE) {
                     const uint32_t n = min((uint32_t)STAGE, E1 - s0);
                     __syncthreads();
                     for (uint32_t k = lane; k < n; k += WAVE) {
