@@ -50,6 +50,10 @@ def algorithmic_bytes(batch, n_tiles, n_spans):
     return b_in, b_out
 
 
+# Above this many verbs the CPU baseline is a sample and has no full result.
+CPU_FULL_MAX_VERBS = 2_000_000
+
+
 def cpu_baseline(batch, seconds):
     """The oracle (a sequential C port of the reference Rasterizer) on one
     host core, repeated over the same batch until `seconds` have elapsed.  A
@@ -63,7 +67,7 @@ def cpu_baseline(batch, seconds):
     except Exception:
         pass
     oracle_ref.load()
-    if batch.verbs.size > 2_000_000:
+    if batch.verbs.size > CPU_FULL_MAX_VERBS:
         t0 = time.perf_counter()
         done = 0
         while done < batch.n_paths and (time.perf_counter() - t0 < seconds or done == 0):
@@ -194,6 +198,8 @@ def main():
                     help="batched pipeline parameters (waves/CU, paths per pop, tiles per chunk) instead of autotuning")
     ap.add_argument("--dense", default=None, metavar="G,W",
                     help="dense kernel launch shape (workgroups per CU, waves per workgroup) instead of autotuning")
+    ap.add_argument("--dense-small", type=int, default=None, metavar="G",
+                    help="workgroups per CU of the small-grid dense instance (0: off) instead of autotuning")
     ap.add_argument("--no-dense", action="store_true", help="the ordered batched kernel alone (bit-exact sums)")
     ap.add_argument("--no-graph", action="store_true", help="launch the batched kernel directly, not from a graph")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -260,6 +266,9 @@ def main():
         from ochre_amd.device import autotune
         dg, dw = (int(x) for x in args.dense.split(",")) if args.dense else (4, 4)
         set_dense_params(ctx, not args.no_dense, dg, dw)
+        if args.dense_small is not None:
+            from ochre_amd.device import set_dense_small
+            set_dense_small(ctx, args.dense_small)
         if args.params or (args.dense and not args.no_dense):
             w, g, tc = (int(x) for x in (args.params or "8,1,32").split(","))
             set_batched_params(ctx, w, g, tc, graph=not args.no_graph)
@@ -311,7 +320,8 @@ def main():
                                graph=bool(bp["graph"]))
             kname = "k_dense" if bp.get("dense") else "k_fused"
         torch.cuda.synchronize()
-        outputs[mode] = out.to_host()
+        if not args.no_cpu and batch.verbs.size <= CPU_FULL_MAX_VERBS:  # checked against the oracle below
+            outputs[mode] = out.to_host()
         res[mode] = {"ms": ev_ms, "wall_ms": wall_ms, "kernel": kname, "kernel_ms": kernels.get(kname, 0.0),
                      "kernels_ms": kernels}
         log(f"{mode}: {ev_ms:.3f} ms/step (wall {wall_ms:.3f}), {batch.n_paths / ev_ms * 1e3:,.0f} paths/s, "

@@ -58,7 +58,7 @@ enum ochre_status {
     OCHRE_ERR_FLATTEN_LIMIT = -5, /* a curve needed > 65536 lines or conic depth > 32
                                      (inputs on which the reference does not terminate) */
     OCHRE_ERR_NO_DEVICE = -6,
-    OCHRE_ERR_TOO_LARGE = -7      /* a batch exceeds 32-bit internal indexing */
+    OCHRE_ERR_TOO_LARGE = -7      /* one path exceeds the per-pass limits (internal 32-bit indexing) */
 };
 
 /* Transform (src/geom.rs:203-207): m is the row-major Mat2x2
@@ -225,9 +225,20 @@ int ochre_context_get_dense_params(ochre_context* ctx, int* enabled, int* groups
 /* A small-grid dense instance (tile grids of at most 32 tiles, 2 waves and
  * 16 KB of LDS per workgroup, so twice as many paths in flight per CU) runs in
  * front of the 64-tile one and hands it the larger paths; groups_per_cu 0
- * (off) to 9, default 8. */
+ * (off) to 9, default 0 (on the 10,000-glyph run one 64-tile instance is
+ * faster; the autotuner tries both). */
 int ochre_context_set_dense_small(ochre_context* ctx, int groups_per_cu);
 int ochre_context_get_dense_small(ochre_context* ctx, int* groups_per_cu);
+
+/* Per-pass limits of the plain pipeline: one pass holds at most max_lines
+ * flattened lines and max_entries sort entries (defaults 2^29 and 2^30, about
+ * 66 GB of intermediates at both).  A batch over either is rasterised as
+ * consecutive path ranges into the same output, with the same result; a
+ * single path over them returns OCHRE_ERR_TOO_LARGE.  0 restores a default;
+ * values above the defaults are clamped to them.  Tests lower the limits to
+ * exercise the split on small batches.  No reference counterpart (the
+ * reference holds every increment of one path in host memory). */
+int ochre_context_set_pass_limits(ochre_context* ctx, int64_t max_lines, int64_t max_entries);
 
 /* Coverage adjoint.  The loss of path p is the sum, over the pixels of the
  * union of its tiles and its target's tiles, of (cov - target)^2, where a

@@ -29,7 +29,7 @@ EXPORTS = [
     "ochre_context_get_batched_params", "ochre_autotune_batched", "ochre_coverage_adjoint",
     "ochre_rasterizer_stroke", "ochre_stroke_plain", "ochre_stroke_batched",
     "ochre_context_set_dense_params", "ochre_context_get_dense_params", "ochre_build_atlas",
-    "ochre_context_set_dense_small", "ochre_context_get_dense_small",
+    "ochre_context_set_dense_small", "ochre_context_get_dense_small", "ochre_context_set_pass_limits",
 ]
 
 
@@ -130,6 +130,8 @@ def _optional(lib):
         lib.ochre_context_get_dense_params.argtypes = [P] + [ctypes.POINTER(I)] * 3
         lib.ochre_context_set_dense_small.argtypes = [P, I]
         lib.ochre_context_get_dense_small.argtypes = [P, ctypes.POINTER(I)]
+    if hasattr(lib, "ochre_context_set_pass_limits"):
+        lib.ochre_context_set_pass_limits.argtypes = [P, ctypes.c_int64, ctypes.c_int64]
     if hasattr(lib, "ochre_build_atlas"):
         i64 = ctypes.c_int64
         lib.ochre_build_atlas.argtypes = [P, ctypes.POINTER(BatchOutputC), i64, P, P, ctypes.c_int, P, i64, P, i64,

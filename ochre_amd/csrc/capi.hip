@@ -264,6 +264,14 @@ int ochre_context_set_dense_small(ochre_context* ctx, int groups_per_cu) {
     return OCHRE_OK;
 }
 
+int ochre_context_set_pass_limits(ochre_context* ctx, int64_t max_lines, int64_t max_entries) {
+    if (!ctx || max_lines < 0 || max_entries < 0) return OCHRE_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->c.max_lines = (max_lines == 0) ? PLAIN_MAX_LINES : std::min<int64_t>(max_lines, PLAIN_MAX_LINES);
+    ctx->c.max_entries = (max_entries == 0) ? PLAIN_MAX_ENTRIES : std::min<int64_t>(max_entries, PLAIN_MAX_ENTRIES);
+    return OCHRE_OK;
+}
+
 int ochre_context_get_dense_small(ochre_context* ctx, int* groups_per_cu) {
     if (!ctx || !groups_per_cu) return OCHRE_ERR_INVALID_ARG;
     std::lock_guard<std::mutex> lk(ctx->mu);

@@ -46,6 +46,9 @@ struct OutputDesc {
     int64_t* path_span_start = nullptr;
     int32_t* path_span_count = nullptr;
     int64_t n_tiles = 0, n_spans = 0;
+    // added to the per-path starts written (the output arrays above already
+    // point at tile_base / span_base of the caller's arrays)
+    int64_t tile_base = 0, span_base = 0;
 };
 
 // Per-path tile grid: keys are ((row_base + ty - ty0) << xbits) | (tx - tx0).
@@ -66,7 +69,8 @@ struct PathBox {
 struct Counters {
     uint32_t error;        // OR of error bits
     uint32_t max_w;        // max grid width in tiles
-    uint32_t pad[62];
+    unsigned long long entries;  // sort entries of all lines (64-bit: a pass over 2^32 is detected)
+    uint32_t pad[60];
 };
 
 enum ErrorBits : uint32_t {
@@ -101,6 +105,15 @@ struct AtlasState {
     ~AtlasState();
 };
 
+// The plain five-kernel pipeline.  One pass holds at most PLAIN_MAX_LINES
+// flattened lines and PLAIN_MAX_ENTRIES sort entries (about 66 GB of
+// intermediates at both limits); a batch of independent paths above either is
+// split into path ranges that are rasterised one after another into the same
+// output (same result: paths are independent).  A single path above them, or
+// a segment input (several fills per path), returns -7.
+constexpr int64_t PLAIN_MAX_LINES = 1ll << 29;
+constexpr int64_t PLAIN_MAX_ENTRIES = 1ll << 30;
+
 struct Context {
     int device = 0;
     hipStream_t own_stream = nullptr;
@@ -114,6 +127,9 @@ struct Context {
     DBuf keys[2], vals[2], records, rs_counts;
     DBuf grp_begin, grp_key, grp_hasrun, grp_sign;
     DBuf row_gbegin, row_gend, row_sign, row_wb, row_path, row_cnt, row_off;
+    DBuf split_off;  // rebased per-path offsets of a path range (run_plain's split)
+    double split_ratio = 2.0;  // how far the last pass that had to split was over its limits
+    int64_t max_lines = PLAIN_MAX_LINES, max_entries = PLAIN_MAX_ENTRIES;  // per-pass limits (lowered by tests)
     // host staging
     void* host_pinned = nullptr;
     // timing
@@ -131,12 +147,13 @@ struct Context {
     bool dense = true;
     int dense_groups_per_cu = 4;
     int dense_waves = 4;
-    int dense_small_groups = 8;  // small-grid instance in front (0: off)
+    int dense_small_groups = 0;  // small-grid instance in front (0: off)
     StrokeBuffers stroke;
     AtlasState atlas;
     ~Context();
 };
 
+// The plain five-kernel pipeline; see the per-pass limits above.
 int run_plain(Context* ctx, const SegInput& in, OutputDesc& out, hipStream_t stream);
 // Persistent work-queue pipeline (one wave per path); paths that do not fit a
 // wave are deferred to run_plain.  waves_per_cu sets the persistent grid,

@@ -14,7 +14,9 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cmath>
 #include <cstring>
+#include <vector>
 
 #include "device_common.hpp"
 #include "pipeline.hpp"
@@ -385,14 +387,28 @@ struct DdaArgs {
 template <bool EMIT>
 __global__ __launch_bounds__(256) void k_dda(DdaArgs a) {
     int64_t l = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (!EMIT) {
+        unsigned long long e = 0;
+        if (l < a.n_lines) {
+            float4 L = a.lines[l];
+            CountSink s;
+            s.cnt = a.cnt;
+            walk_line(mk(L.x, L.y), mk(L.z, L.w), s);
+            a.line_cnt[l] = ((uint64_t)s.runs << 32) | s.entries;
+            e = s.entries;
+        }
+        // the 64-bit total of the block's entries: the packed per-line counts
+        // are scanned in 32-bit halves, which wrap silently past 2^32
+        for (int o = 32; o > 0; o >>= 1) e += __shfl_xor(e, o);
+        __shared__ unsigned long long part[4];
+        if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = e;
+        __syncthreads();
+        if (threadIdx.x == 0) atomicAdd(&a.cnt->entries, part[0] + part[1] + part[2] + part[3]);
+        return;
+    }
     if (l >= a.n_lines) return;
     float4 L = a.lines[l];
-    if (!EMIT) {
-        CountSink s;
-        s.cnt = a.cnt;
-        walk_line(mk(L.x, L.y), mk(L.z, L.w), s);
-        a.line_cnt[l] = ((uint64_t)s.runs << 32) | s.entries;
-    } else {
+    {
         uint64_t off = a.line_cnt[l];
         EmitSink s;
         s.line = (uint32_t)l;
@@ -813,16 +829,16 @@ __global__ __launch_bounds__(WAVE) void k_tiles(TileArgs a) {
 // Per-path output ranges and the empty-path tile.
 __global__ void k_path_out(const PathGrid* grid, int64_t n_paths, const uint64_t* row_off, const uint64_t* total,
                            int64_t* tstart, int32_t* tcount, int64_t* sstart, int32_t* scount, int16_t* tile_xy,
-                           uint8_t* tile_alpha) {
+                           uint8_t* tile_alpha, int64_t tile_base, int64_t span_base) {
     int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (p >= n_paths) return;
     PathGrid g = grid[p];
     uint64_t b = row_off[g.row_base];
     uint64_t e = (p + 1 < n_paths) ? row_off[grid[p + 1].row_base] : *total;
     uint64_t fb = row_off[g.row_base + g.h];
-    tstart[p] = (int64_t)(b >> 32);
+    tstart[p] = tile_base + (int64_t)(b >> 32);
     tcount[p] = (int32_t)((e >> 32) - (b >> 32));
-    sstart[p] = (int64_t)(uint32_t)b;
+    sstart[p] = span_base + (int64_t)(uint32_t)b;
     scount[p] = (int32_t)((uint32_t)e - (uint32_t)b);
     if ((e >> 32) - (fb >> 32) == 1) {  // the empty-path tile
         uint64_t ti = fb >> 32;
@@ -848,16 +864,23 @@ struct HostTotals {
     uint32_t groups;
     uint32_t pad;
     uint64_t out;     // (tiles << 32) | spans
+    uint64_t entries;
 };
 
-int run_plain(Context* ctx, const SegInput& in, OutputDesc& out, hipStream_t s) {
+// Internal status of one pass: the input is over the per-pass limits.
+constexpr int ST_SPLIT = -100;
+
+static int run_plain_pass(Context* ctx, const SegInput& in, OutputDesc& out, hipStream_t s) {
     const int64_t V = in.n_verbs;
     const int64_t NP = in.n_paths;
     if (NP <= 0) {
         out.n_tiles = out.n_spans = 0;
         return 0;
     }
-    if (V >= (1ll << 31) || in.n_points >= (1ll << 32)) return -7;
+    if (V >= (1ll << 31) || in.n_points >= (1ll << 32)) {
+        ctx->split_ratio = std::max((double)V / (double)(1ll << 31), (double)in.n_points / (double)(1ll << 32));
+        return ST_SPLIT;
+    }
     HIPCHK(ctx->counters.ensure(256));
     HIPCHK(ctx->totals.ensure(256));
     if (!ctx->host_pinned) HIPCHK(hipHostMalloc(&ctx->host_pinned, 4096));
@@ -909,6 +932,10 @@ int run_plain(Context* ctx, const SegInput& in, OutputDesc& out, hipStream_t s) 
     if (ht->error & ERR_INPUT) return -1;
     if (ht->error & ERR_FLATTEN) return -5;
     const int64_t L = ht->lines;
+    if (L > ctx->max_lines) {
+        ctx->split_ratio = (double)L / (double)ctx->max_lines;
+        return ST_SPLIT;
+    }
 
     HIPCHK(ctx->lines.ensure(std::max<int64_t>(L, 1) * 16));
     HIPCHK(ctx->line_path.ensure(std::max<int64_t>(L, 1) * 4));
@@ -941,14 +968,24 @@ int run_plain(Context* ctx, const SegInput& in, OutputDesc& out, hipStream_t s) 
     HIPCHK(hipMemcpyAsync(&ht->dda, &tot[2], 8, hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(&ht->max_w, &cnt->max_w, 4, hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(&ht->error, &cnt->error, 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(&ht->entries, &cnt->entries, 8, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     if (ht->error & ERR_FLATTEN) return -5;
+    // entries >= runs (every run record is also a sort entry), so below the
+    // limit both 32-bit halves of the packed total are exact
+    if (ht->entries > (uint64_t)ctx->max_entries) {
+        ctx->split_ratio = (double)ht->entries / (double)ctx->max_entries;
+        return ST_SPLIT;
+    }
     const int64_t R = (int64_t)(uint32_t)ht->dda;
     const int64_t RUNS = (int64_t)(ht->dda >> 32);
     const int64_t ROWS = ht->rows;
     const uint32_t xbits = std::max<uint32_t>(1, bits_for(std::max<uint32_t>(ht->max_w, 1)));
     const uint32_t rbits = bits_for((uint64_t)ROWS);
-    if (xbits + rbits > 32 || L >= (1ll << 31) || RUNS >= (1ll << 31)) return -7;
+    if (xbits + rbits > 32) {
+        ctx->split_ratio = 2.0;
+        return ST_SPLIT;
+    }
     const uint32_t kbits = xbits + rbits;
 
     // ---- (b) binning + sort
@@ -1067,7 +1104,7 @@ int run_plain(Context* ctx, const SegInput& in, OutputDesc& out, hipStream_t s) 
     if (ctx->timing) HIPCHK(hipEventRecord(ctx->ev[1], s));
     hipLaunchKernelGGL(k_path_out, dim3(grid_for(NP, 256)), dim3(256), 0, s, grid, NP, ta.row_off,
                        (const uint64_t*)&tot[4], out.path_tile_start, out.path_tile_count, out.path_span_start,
-                       out.path_span_count, out.tile_xy, out.tile_alpha);
+                       out.path_span_count, out.tile_xy, out.tile_alpha, out.tile_base, out.span_base);
     HIPCHK(hipGetLastError());
     if (ctx->timing) {
         HIPCHK(hipEventSynchronize(ctx->ev[1]));
@@ -1078,6 +1115,102 @@ int run_plain(Context* ctx, const SegInput& in, OutputDesc& out, hipStream_t s) 
         t.second += 1;
     }
     return 0;
+}
+
+// Offsets of paths [p0, p0 + n] rebased to the range's first verb, point and
+// weight, packed as three arrays of n + 1.
+__global__ void k_rebase_offsets(const int64_t* voff, const int64_t* poff, const int64_t* woff, int64_t p0, int64_t n,
+                                 int64_t* dst) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i > n) return;
+    dst[i] = voff[p0 + i] - voff[p0];
+    dst[n + 1 + i] = poff[p0 + i] - poff[p0];
+    dst[2 * (n + 1) + i] = woff[p0 + i] - woff[p0];
+}
+
+int run_plain(Context* ctx, const SegInput& in, OutputDesc& out, hipStream_t s) {
+    int st = run_plain_pass(ctx, in, out, s);
+    if (st != ST_SPLIT) return st;
+    // Paths are independent Rasterizers: rasterise path ranges one after
+    // another, each range's tiles and spans following the previous range's.
+    if (in.path_seg_off || in.seg_path || in.n_paths < 2) return -7;
+    const int64_t NP = in.n_paths;
+    std::vector<int64_t> voff(NP + 1), poff(NP + 1), woff(NP + 1);
+    HIPCHK(hipMemcpyAsync(voff.data(), in.seg_verb_off, (NP + 1) * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(poff.data(), in.seg_point_off, (NP + 1) * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(woff.data(), in.seg_weight_off, (NP + 1) * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(ctx->split_off.ensure((size_t)3 * (NP + 1) * 8));
+    int64_t* roff = ctx->split_off.as<int64_t>();
+
+    // ranges still to do, the next one at the back; a range over the limits is
+    // cut into pieces of about equal verb counts, sized from how far the
+    // failed pass was over
+    std::vector<std::pair<int64_t, int64_t>> todo;
+    auto cut = [&](int64_t p0, int64_t p1) {
+        const int k = (int)std::min<double>(std::ceil(std::max(2.0, 1.25 * ctx->split_ratio)), (double)(p1 - p0));
+        std::vector<int64_t> cuts{p0};
+        const int64_t v0 = voff[p0], vr = voff[p1] - v0;
+        for (int i = 1; i < k; i++) {
+            const int64_t target = v0 + (int64_t)((double)vr * i / k);
+            int64_t c = std::lower_bound(voff.begin() + p0, voff.begin() + p1, target) - voff.begin();
+            if (c > cuts.back() && c < p1) cuts.push_back(c);
+        }
+        if (cuts.size() == 1) cuts.push_back(p0 + (p1 - p0) / 2);
+        cuts.push_back(p1);
+        for (size_t i = cuts.size() - 1; i > 0; i--) todo.push_back({cuts[i - 1], cuts[i]});
+    };
+    cut(0, NP);
+    int64_t T = 0, S = 0;
+    bool room = true;
+    while (!todo.empty()) {
+        const int64_t p0 = todo.back().first, p1 = todo.back().second;
+        todo.pop_back();
+        const int64_t n = p1 - p0;
+        hipLaunchKernelGGL(k_rebase_offsets, dim3(grid_for(n + 1, 256)), dim3(256), 0, s, in.seg_verb_off,
+                           in.seg_point_off, in.seg_weight_off, p0, n, roff);
+        HIPCHK(hipGetLastError());
+        SegInput si;
+        si.n_segs = n;
+        si.n_paths = n;
+        si.verbs = in.verbs + voff[p0];
+        si.points = in.points + 2 * poff[p0];
+        si.weights = in.weights ? in.weights + woff[p0] : nullptr;
+        si.seg_verb_off = roff;
+        si.seg_point_off = roff + (n + 1);
+        si.seg_weight_off = roff + 2 * (n + 1);
+        si.seg_xf = in.seg_xf + p0;
+        si.seg_raw = in.seg_raw ? in.seg_raw + p0 : nullptr;
+        si.n_verbs = voff[p1] - voff[p0];
+        si.n_points = poff[p1] - poff[p0];
+        si.n_weights = woff[p1] - woff[p0];
+        OutputDesc od;
+        od.tile_xy = out.tile_xy + (room ? 2 * T : 0);
+        od.tile_alpha = out.tile_alpha + (room ? 64 * T : 0);
+        od.tile_capacity = room ? out.tile_capacity - T : 0;
+        od.span_xy = out.span_xy + (room ? 2 * S : 0);
+        od.span_w = out.span_w + (room ? S : 0);
+        od.span_capacity = room ? out.span_capacity - S : 0;
+        od.path_tile_start = out.path_tile_start + p0;
+        od.path_tile_count = out.path_tile_count + p0;
+        od.path_span_start = out.path_span_start + p0;
+        od.path_span_count = out.path_span_count + p0;
+        od.tile_base = out.tile_base + T;
+        od.span_base = out.span_base + S;
+        st = run_plain_pass(ctx, si, od, s);
+        if (st == ST_SPLIT) {
+            if (n < 2) return -7;
+            cut(p0, p1);
+            continue;
+        }
+        if (st == -4) room = false;
+        else if (st) return st;
+        T += od.n_tiles;
+        S += od.n_spans;
+    }
+    out.n_tiles = T;
+    out.n_spans = S;
+    return (room && T <= out.tile_capacity && S <= out.span_capacity) ? 0 : -4;
 }
 
 }  // namespace ochre

@@ -274,7 +274,13 @@ def dense_params(ctx: Context):
             "dense_small_groups_per_cu": sg.value}
 
 
-def set_dense_small(ctx: Context, groups_per_cu=8):
+def set_pass_limits(ctx: Context, max_lines=0, max_entries=0):
+    """Per-pass limits of the plain pipeline (0: default); a batch over them
+    is rasterised as consecutive path ranges."""
+    _lib.check(ctx.lib.ochre_context_set_pass_limits(ctx.h, int(max_lines), int(max_entries)), "set_pass_limits")
+
+
+def set_dense_small(ctx: Context, groups_per_cu=0):
     """The small-grid dense instance in front of the 64-tile one (0: off)."""
     _lib.check(ctx.lib.ochre_context_set_dense_small(ctx.h, groups_per_cu), "set_dense_small")
 
@@ -308,7 +314,7 @@ def autotune(ctx: Context, db: DeviceBatch, out: DeviceOutput, reps=3, cache_key
             if "dense" in cached:
                 set_dense_params(ctx, bool(cached["dense"]), cached["dense_groups_per_cu"],
                                  cached["dense_waves_per_group"])
-                set_dense_small(ctx, cached.get("dense_small_groups_per_cu", 8))
+                set_dense_small(ctx, cached.get("dense_small_groups_per_cu", 0))
             return cached, []
     res = (_lib.BatchedTuningC * 64)()
     n = ctypes.c_int()

@@ -64,6 +64,70 @@ def test_star_blobs_small(gpu_ctx, mode):
     check(gpu_ctx, b.subset(order[:3]), mode, max_lsb=0)
 
 
+_SCENES_REF = {}
+
+
+def scenes_small():
+    """The batched-scenes generator at 8 scenes (1,426 paths, 79 blobs of
+    1k-5k cubics, 1.8 M tiles): every structural trait of the full config
+    (log-uniform paths per scene, blobs handed on to the plain pipeline,
+    rotated and sheared transforms) at a size the oracle renders in seconds."""
+    b = W.scenes(n_scenes=8, seed=12, nmax=5e3)
+    if "ref" not in _SCENES_REF:
+        _SCENES_REF["ref"] = oracle_ref.run_batch(b)
+    return b, _SCENES_REF["ref"]
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_scenes_small(gpu_ctx, mode):
+    from ochre_amd.device import run_to_host
+    b, ref = scenes_small()
+    print(f"[{mode}] {b.name}", flush=True)
+    got = run_to_host(gpu_ctx, b, mode, tile_capacity=ref.n_tiles + 64, span_capacity=ref.n_spans + 64)
+    st = compare(ref, got, max_lsb=1 if mode == "dense" else 0)
+    print(f"[{mode}] {b.name}: {st}", flush=True)
+    if mode == "dense":
+        assert st["bit_exact_fraction"] >= 0.999
+
+
+@pytest.mark.parametrize("mode", ["plain", "batched"])
+@pytest.mark.parametrize("limits", [(600000, 0), (0, 4000000), (300000, 2500000)])
+def test_plain_split_into_path_ranges(gpu_ctx, mode, limits):
+    """Per-pass limits lowered so that the plain pipeline (run directly, or
+    on the paths the batched kernels hand on) cuts the batch into path
+    ranges: the output must still be the oracle's, with per-path ranges
+    pointing into one contiguous tile and span array.  The batch has 9.6 M
+    lines (at most 251 k in one path) and at most 68 M sort entries (1.9 M in
+    one path), so every limit here cuts it into several ranges while each
+    path still fits one pass.  The first attempt runs out of output room
+    (capacity 1024), so the ranges after the overflow only count."""
+    from ochre_amd.device import run_to_host, set_pass_limits
+    b, ref = scenes_small()
+    print(f"[{mode}] {b.name} limits={limits}", flush=True)
+    set_pass_limits(gpu_ctx, *limits)
+    try:
+        got = run_to_host(gpu_ctx, b, mode, tile_capacity=1024, span_capacity=1024)  # grown once, after -4
+    finally:
+        set_pass_limits(gpu_ctx, 0, 0)
+    st = compare(ref, got, max_lsb=0)
+    print(f"[{mode}] split {limits}: {st}", flush=True)
+
+
+def test_plain_single_path_over_pass_limits(gpu_ctx):
+    """A single path over the per-pass limits cannot be split: TOO_LARGE."""
+    from ochre_amd import _lib
+    from ochre_amd.device import run_to_host, set_pass_limits
+    b = W.star_blobs()
+    b = b.subset(np.argsort(np.diff(b.verb_off))[:1])
+    set_pass_limits(gpu_ctx, 1000, 0)
+    try:
+        with pytest.raises(_lib.OchreError) as e:
+            run_to_host(gpu_ctx, b, "plain")
+    finally:
+        set_pass_limits(gpu_ctx, 0, 0)
+    assert e.value.status == -7
+
+
 @pytest.mark.parametrize("mode", MODES)
 def test_adjoint_glyphs(gpu_ctx, mode):
     check(gpu_ctx, W.adjoint_glyphs(), mode, max_lsb=0)
@@ -184,4 +248,4 @@ def test_batched_params_and_graph_replays(gpu_ctx):
     finally:
         lib.ochre_context_set_batched_params(gpu_ctx.h, 6, 2, 32, 1)
         lib.ochre_context_set_dense_params(gpu_ctx.h, 1, 4, 4)
-        lib.ochre_context_set_dense_small(gpu_ctx.h, 8)
+        lib.ochre_context_set_dense_small(gpu_ctx.h, 0)
