@@ -296,10 +296,10 @@ def main():
             fn = lambda: run_plain(out)  # noqa: E731
             ctx.set_timing(True)
             ev_ms, wall_ms = time_steps(fn, args.steps, args.warmup)
-            kms, kn = ctx.timing("k_tiles")
+            kms, kn = ctx.timing("k_tile_rows")
             ctx.set_timing(False)
-            kernels = {"k_tiles": kms / max(kn, 1)}
-            kname = "k_tiles"
+            kernels = {"k_tile_rows": kms / max(kn, 1)}
+            kname = "k_tile_rows"
         else:
             fn = lambda: run_batched(out)  # noqa: E731
             # the production launch (graph replay unless --no-graph)

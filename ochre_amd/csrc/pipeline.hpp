@@ -121,7 +121,7 @@ struct Context {
     DBuf scan_partials;
     // pipeline scratch
     DBuf cmd_scan, cmd_seg, cmd_lines, cmd_line_off;
-    DBuf lines, line_path, line_cnt, line_off;
+    DBuf lines, line_path, line_cnt, line_off, dda_blk;
     DBuf path_box, path_grid, path_rows;
     DBuf counters, totals;
     DBuf keys[2], vals[2], records, rs_counts;

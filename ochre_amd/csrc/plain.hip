@@ -4,11 +4,14 @@
 //   (b) binning      k_dda<false>/<true> walk each line's DDA and emit one
 //                    sort entry per tile run and per tile-row crossing,
 //                    keyed (path, tile_y, tile_x); k_rs_* radix-sort them
-//   (c) accumulation k_tiles<true>: one wave per tile row, one lane per tile,
-//                    runs replayed in emission order into LDS accumulators
-//   (d) row scan     the carry (rasterizer.rs:233-250) and the winding of the
-//                    tile-row crossings (rasterizer.rs:252-261)
-//   (e) compaction   tiles and spans written at scanned offsets
+//   (c) accumulation k_tile_rows: one wave per tile row; a lane per run
+//                    replays the row's runs, then the runs are added in
+//                    sorted order, a lane per increment, into LDS sums
+//   (d) row scan     the carry (rasterizer.rs:233-250) folded across the
+//                    lanes of each pixel row, and the winding of the tile-row
+//                    crossings (rasterizer.rs:252-261)
+//   (e) compaction   k_tile_counts counts tiles and spans per tile row; they
+//                    are written at the scanned offsets
 // References: glowcoil/ochre src/path.rs:41-109, src/rasterizer.rs:61-268.
 #include <hip/hip_runtime.h>
 
@@ -382,6 +385,7 @@ struct DdaArgs {
     uint32_t* vals;
     RunRec* recs;
     Counters* cnt;
+    uint64_t* blk_entries;     // count pass: the entries of each block's lines
 };
 
 template <bool EMIT>
@@ -397,13 +401,14 @@ __global__ __launch_bounds__(256) void k_dda(DdaArgs a) {
             a.line_cnt[l] = ((uint64_t)s.runs << 32) | s.entries;
             e = s.entries;
         }
-        // the 64-bit total of the block's entries: the packed per-line counts
-        // are scanned in 32-bit halves, which wrap silently past 2^32
+        // the 64-bit total of the block's entries (k_sum_u64 adds the blocks):
+        // the packed per-line counts are scanned in 32-bit halves, which wrap
+        // silently past 2^32
         for (int o = 32; o > 0; o >>= 1) e += __shfl_xor(e, o);
         __shared__ unsigned long long part[4];
         if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = e;
         __syncthreads();
-        if (threadIdx.x == 0) atomicAdd(&a.cnt->entries, part[0] + part[1] + part[2] + part[3]);
+        if (threadIdx.x == 0) a.blk_entries[blockIdx.x] = part[0] + part[1] + part[2] + part[3];
         return;
     }
     if (l >= a.n_lines) return;
@@ -422,6 +427,18 @@ __global__ __launch_bounds__(256) void k_dda(DdaArgs a) {
         s.cnt = a.cnt;
         walk_line(mk(L.x, L.y), mk(L.z, L.w), s);
     }
+}
+
+// *out += sum of a[0, n): a grid of at most 256 blocks, one atomic each.
+__global__ __launch_bounds__(256) void k_sum_u64(const uint64_t* a, int64_t n, unsigned long long* out) {
+    unsigned long long v = 0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        v += a[i];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    __shared__ unsigned long long part[4];
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(out, part[0] + part[1] + part[2] + part[3]);
 }
 
 struct LoadU64 {
@@ -536,39 +553,8 @@ struct TileArgs {
     Counters* cnt;
 };
 
-constexpr int REPLAY_LIMIT = 64;  // a tile run has at most 15 increments
-constexpr int TG = 32;             // tiles per chunk (one lane each)
-constexpr int TSTR = 65;           // accumulator stride per tile (spreads LDS banks)
+constexpr int TG = 32;             // groups per chunk (one lane each)
 constexpr int ROWS_PER_WAVE = 8;   // tile rows per wave
-constexpr int STAGE = 256;         // run records staged in LDS per pass
-
-// Replays one tile run into the accumulator of tile slot `t`.
-__device__ inline void replay_run(const RunRec& r, const float4& L, int16_t tx, int16_t ty, float* acc_a,
-                                  float* acc_h, int t, Counters* cnt) {
-    Dda d;
-    V2 last = mk(L.x, L.y), point = mk(L.z, L.w);
-    if (r.line >> 31) {
-        dda_init(d, last, point);
-    } else {
-        dda_consts(d, last, point);
-        d.x = (int16_t)(r.xy & 0xffffu);
-        d.y = (int16_t)(r.xy >> 16);
-        d.row_t1 = r.row_t1;
-        d.col_t1 = r.col_t1;
-        d.t0 = 0.0f;
-        dda_step(d);
-    }
-    for (int k = 0; k < REPLAY_LIMIT; k++) {
-        float area, height;
-        dda_increment(d, area, height);
-        int pix = ((d.y & 7) << 3) | (d.x & 7);
-        acc_a[t * TSTR + pix] += area;
-        acc_h[t * TSTR + pix] += height;
-        if (dda_step(d)) return;
-        if (div8(d.x) != tx || div8(d.y) != ty) return;
-    }
-    atomicOr(&cnt->error, (uint32_t)ERR_REPLAY);
-}
 
 // Inclusive scan restarting at every lane with head set.
 __device__ inline int32_t seg_incl_scan(int32_t v, bool head) {
@@ -584,18 +570,10 @@ __device__ inline int32_t seg_incl_scan(int32_t v, bool head) {
     return v;
 }
 
-// One wave per block of 8 consecutive tile rows; each chunk of up to 32
-// tiles (groups) is handled one lane per tile.  The count pass only counts
-// tiles and spans per row; the emit pass also accumulates, carries along
-// rows and writes.
-template <bool EMIT>
-__global__ __launch_bounds__(WAVE) void k_tiles(TileArgs a) {
-    __shared__ float acc_a[EMIT ? TG * TSTR : 1];
-    __shared__ float acc_h[EMIT ? TG * TSTR : 1];
-    __shared__ float cin[EMIT ? TG * 8 : 1];
-    __shared__ float4 st_line[EMIT ? STAGE : 1];
-    __shared__ RunRec st_rec[EMIT ? STAGE : 1];
-    __shared__ uint64_t slot_tiles[ROWS_PER_WAVE];
+// Count pass of the tile stage: tiles and spans per tile row.  One wave per
+// block of 8 consecutive tile rows; each chunk of up to 32 groups is handled
+// one lane per group, the rows' running winding carried between chunks.
+__global__ __launch_bounds__(WAVE) void k_tile_counts(TileArgs a) {
     // per row slot: running winding, tiles, spans, pending tile (for spans)
     __shared__ int32_t r_w[ROWS_PER_WAVE], r_pend_tx[ROWS_PER_WAVE], r_pend_w[ROWS_PER_WAVE];
     __shared__ uint32_t r_nt[ROWS_PER_WAVE], r_ns[ROWS_PER_WAVE], r_pend[ROWS_PER_WAVE];
@@ -634,7 +612,6 @@ __global__ __launch_bounds__(WAVE) void k_tiles(TileArgs a) {
         }
         G0 = __shfl(G0, 0, WAVE);
         G1 = __shfl(G1, 0, WAVE);
-        float carry = 0.0f;  // lane = slot * 8 + pixel row
         __syncthreads();
         for (uint32_t c = G0; c < G1 && G0 < G1; c += TG) {
             const uint32_t g = c + lane;
@@ -642,17 +619,10 @@ __global__ __launch_bounds__(WAVE) void k_tiles(TileArgs a) {
             const uint32_t key = valid ? a.gkey[g] : 0u;
             const int64_t row = (int64_t)(key >> a.xbits);
             const int slot = valid ? (int)(row - R0) : ROWS_PER_WAVE;
-            int32_t tx = 0, ty = 0;
-            uint32_t row_tiles = 0, row_spans = 0;
+            int32_t tx = 0;
             if (valid) {
                 const PathGrid G = a.grid[a.row_path[row]];
                 tx = G.tx0 + (int32_t)(key & xmask);
-                ty = G.ty0 + (int32_t)(row - (int64_t)G.row_base);
-                if (EMIT) {
-                    uint64_t off = a.row_off[row];
-                    row_tiles = (uint32_t)(off >> 32);
-                    row_spans = (uint32_t)off;
-                }
             }
             const bool hr = valid && a.ghasrun[g];
             const int32_t sg = valid ? a.gsign[g] : 0;
@@ -669,7 +639,6 @@ __global__ __launch_bounds__(WAVE) void k_tiles(TileArgs a) {
             const uint64_t tiles = __ballot(hr);
             const int32_t wend = (valid ? r_w[slot] : 0) + seg_incl_scan(sg, head);
             const uint64_t my_tiles = tiles & same;
-            const uint32_t rank = (valid ? r_nt[slot] : 0u) + (uint32_t)__popcll(my_tiles & lt);
             // spans to the next tile of the same row (rasterizer.rs:252-264)
             const uint64_t above = my_tiles & ~le;
             const int nb = above ? (__ffsll((long long)above) - 1) : lane;
@@ -678,129 +647,9 @@ __global__ __launch_bounds__(WAVE) void k_tiles(TileArgs a) {
             // the first tile of a row in this chunk closes that row's pending tile
             const bool first_tile = hr && (my_tiles & lt) == 0;
             bool pend_span = false;
-            int32_t ptx = 0;
-            if (first_tile && r_pend[slot]) {
-                ptx = r_pend_tx[slot];
-                pend_span = tx > ptx + 1 && r_pend_w[slot] != 0;
-            }
+            if (first_tile && r_pend[slot]) pend_span = tx > r_pend_tx[slot] + 1 && r_pend_w[slot] != 0;
             const uint64_t spans = __ballot(span_here);
             const uint64_t pends = __ballot(pend_span);
-            const uint32_t my_spans_before = (uint32_t)__popcll(spans & same & lt) + (uint32_t)__popcll(pends & same);
-            const uint32_t span_base = valid ? r_ns[slot] : 0u;
-            if (EMIT) {
-                if (pend_span) {
-                    const uint32_t si = row_spans + span_base;
-                    a.span_xy[2 * si] = (int16_t)((ptx + 1) * TILE);
-                    a.span_xy[2 * si + 1] = (int16_t)(ty * TILE);
-                    a.span_w[si] = (uint16_t)((uint16_t)(tx - ptx - 1) * (uint16_t)TILE);
-                }
-                if (span_here) {
-                    const uint32_t si = row_spans + span_base + my_spans_before;
-                    a.span_xy[2 * si] = (int16_t)((tx + 1) * TILE);
-                    a.span_xy[2 * si + 1] = (int16_t)(ty * TILE);
-                    a.span_w[si] = (uint16_t)((uint16_t)(btx - tx - 1) * (uint16_t)TILE);
-                }
-            }
-            if (EMIT) {
-                // ---- (c) accumulate: stage run records, replay one lane per tile
-                for (int k = lane; k < TG * TSTR; k += WAVE) {
-                    acc_a[k] = 0.0f;
-                    acc_h[k] = 0.0f;
-                }
-                const uint32_t last_g = min(c + (uint32_t)TG, G1);
-                const uint32_t E0 = a.gbegin[c], E1 = a.gbegin[last_g];
-                const uint32_t my_e0 = hr ? a.gbegin[g] : 0u, my_e1 = hr ? a.gbegin[g + 1] : 0u;
-                // Light chunks stage their run records through LDS together
-                // (coalesced).  Heavy ones (large paths: hundreds of runs per
-                // tile, so one staging block would cover a single tile and the
-                // tiles would be replayed one after another) let every lane
-                // stream its own tile's runs from global memory instead.
-                const bool staged = E1 - E0 <= 2u * (uint32_t)STAGE;
-                if (!staged && hr) {
-                    uint32_t e = my_e0;
-                    uint32_t vn = e < my_e1 ? a.vals[e] : 0xffffffffu;
-                    while (e < my_e1) {
-                        const uint32_t v = vn;
-                        e++;
-                        vn = e < my_e1 ? a.vals[e] : 0xffffffffu;
-                        if (v >> 31) continue;  // a tile-row crossing, not a run
-                        const RunRec r = a.recs[v];
-                        const float4 L = a.lines[r.line & 0x7fffffffu];
-                        replay_run(r, L, (int16_t)tx, (int16_t)ty, acc_a, acc_h, lane, a.cnt);
-                    }
-                }
-                for (uint32_t s0 = E0; staged && s0 < E1; s0 += STAGE) {
-                    const uint32_t n = min((uint32_t)STAGE, E1 - s0);
-                    __syncthreads();
-                    for (uint32_t k = lane; k < n; k += WAVE) {
-                        const uint32_t v = a.vals[s0 + k];
-                        RunRec r;
-                        if (v >> 31) {
-                            r.line = 0xffffffffu;
-                        } else {
-                            r = a.recs[v];
-                            st_line[k] = a.lines[r.line & 0x7fffffffu];
-                        }
-                        st_rec[k] = r;
-                    }
-                    __syncthreads();
-                    if (hr) {
-                        const uint32_t e_lo = max(my_e0, s0), e_hi = min(my_e1, s0 + n);
-                        for (uint32_t e = e_lo; e < e_hi; e++) {
-                            const RunRec r = st_rec[e - s0];
-                            if (r.line == 0xffffffffu) continue;
-                            replay_run(r, st_line[e - s0], (int16_t)tx, (int16_t)ty, acc_a, acc_h, lane, a.cnt);
-                        }
-                    }
-                }
-                // ---- (d) carry along each row: lane = slot * 8 + pixel row
-#pragma unroll
-                for (int rs = 0; rs < ROWS_PER_WAVE; rs++) {  // tiles of each row slot in this chunk
-                    const uint64_t m = __ballot(hr && slot == rs);
-                    if (lane == rs) slot_tiles[rs] = m;
-                }
-                __syncthreads();
-                {
-                    const int cs = lane >> 3, y = lane & 7;
-                    uint64_t m = slot_tiles[cs];
-                    while (m) {
-                        const int t = __ffsll((long long)m) - 1;
-                        m &= m - 1;
-                        cin[t * 8 + y] = carry;
-#pragma unroll
-                        for (int x = 0; x < 8; x++) carry += acc_h[t * TSTR + y * 8 + x];
-                    }
-                }
-                __syncthreads();
-                // ---- (e) quantise and write (rasterizer.rs:231-241)
-                if (hr) {
-                    const uint32_t ti = row_tiles + rank;
-                    uint32_t words[16];
-#pragma unroll
-                    for (int y = 0; y < 8; y++) {
-                        float accum = cin[lane * 8 + y];
-                        uint32_t w0 = 0, w1 = 0;
-#pragma unroll
-                        for (int x = 0; x < 8; x++) {
-                            const int k = lane * TSTR + y * 8 + x;
-                            float v = fminf(fabsf(accum + acc_a[k]) * 256.0f, 255.0f);
-                            uint32_t b = (uint32_t)v;
-                            if (x < 4) w0 |= b << (8 * x);
-                            else w1 |= b << (8 * (x - 4));
-                            accum += acc_h[k];
-                        }
-                        words[2 * y] = w0;
-                        words[2 * y + 1] = w1;
-                    }
-                    uint4* dst = reinterpret_cast<uint4*>(a.tile_alpha + (size_t)ti * 64);
-                    dst[0] = make_uint4(words[0], words[1], words[2], words[3]);
-                    dst[1] = make_uint4(words[4], words[5], words[6], words[7]);
-                    dst[2] = make_uint4(words[8], words[9], words[10], words[11]);
-                    dst[3] = make_uint4(words[12], words[13], words[14], words[15]);
-                    a.tile_xy[2 * ti] = (int16_t)(tx * TILE);
-                    a.tile_xy[2 * ti + 1] = (int16_t)(ty * TILE);
-                }
-            }
             __syncthreads();
             // per-row state after this chunk: the last lane of each row segment
             const int next_slot = __shfl_down(slot, 1, WAVE);
@@ -820,8 +669,164 @@ __global__ __launch_bounds__(WAVE) void k_tiles(TileArgs a) {
             }
             __syncthreads();
         }
-        if (!EMIT && lane < ROWS_PER_WAVE && R0 + lane < a.n_rows && real_row)
+        if (lane < ROWS_PER_WAVE && R0 + lane < a.n_rows && real_row)
             a.row_cnt[R0 + lane] = ((uint64_t)r_nt[lane] << 32) | r_ns[lane];
+        __syncthreads();
+    }
+}
+
+// Emit pass of the tile stage: one wave per tile row, walking the row's
+// groups (bins, rasterizer.rs:213-265) in key order.  The row's sort entries
+// are taken 64 at a time and each lane replays one run into LDS (a run stays
+// inside one 8x8 tile and the DDA never revisits a pixel, so it has at most
+// 15 increments, all at distinct pixels).  The runs are then folded in sorted
+// order, one run per step, lane k adding the run's increment k to its pixel's
+// sums: a wave's LDS operations complete in issue order, so every pixel's f32
+// sums follow the sorted order exactly (rasterizer.rs:221-228).  At the end of
+// a tile, lane = pixel: the row carry is folded across the 8 lanes of each
+// pixel row in x order, the alphas quantised and stored (rasterizer.rs:230-250),
+// and a span is written where the previous tile of the row is more than one
+// tile to the left and the winding after it is non-zero (rasterizer.rs:252-264).
+constexpr int TR_SLOTS = 16;  // increments per run slot
+
+__device__ inline uint32_t rl(uint32_t v, int j) { return (uint32_t)__builtin_amdgcn_readlane((int)v, j); }
+
+__global__ __launch_bounds__(WAVE) void k_tile_rows(TileArgs a) {
+    __shared__ float2 inc[WAVE * TR_SLOTS];
+    __shared__ uint8_t ipix[WAVE * TR_SLOTS];
+    __shared__ float2 acc[64];
+    const int lane = threadIdx.x;
+    const uint32_t xmask = (1u << a.xbits) - 1;
+    acc[lane] = make_float2(0.0f, 0.0f);
+    for (int64_t row = blockIdx.x; row < a.n_rows; row += gridDim.x) {
+        const PathGrid G = a.grid[a.row_path[row]];
+        if (row - (int64_t)G.row_base >= G.h) continue;  // the empty-path row (k_path_out)
+        const uint32_t gb = a.row_gbegin[row], ge = a.row_gend[row];
+        if (ge <= gb) continue;
+        const int32_t ty = G.ty0 + (int32_t)(row - (int64_t)G.row_base);
+        const uint64_t off = a.row_off[row];
+        const uint32_t row_tiles = (uint32_t)(off >> 32), row_spans = (uint32_t)off;
+        const uint32_t E_hi = a.gbegin[ge];
+        int32_t winding = a.row_wb[row];
+        float carry = 0.0f;  // of the lane's pixel row
+        uint32_t nt = 0, ns = 0;
+        bool have_prev = false;
+        int32_t prev_tx = 0, prev_w = 0;
+        // groups gbase + lane: key, has runs, crossing sum, end of its entries
+        uint32_t gbase = gb, gk = 0, ghr = 0, gend = 0;
+        int32_t gsg = 0;
+        auto load_groups = [&]() {
+            const uint32_t g = gbase + (uint32_t)lane;
+            if (g < ge) {
+                gk = a.gkey[g];
+                ghr = a.ghasrun[g];
+                gsg = a.gsign[g];
+                gend = a.gbegin[g + 1];
+            }
+        };
+        load_groups();
+        uint32_t e = a.gbegin[gb];
+        uint32_t ebase = e + WAVE;  // no entry block loaded yet
+        uint32_t nrun = 0;          // increments of the run this lane replayed
+        for (uint32_t g = gb; g < ge; g++) {
+            if (g - gbase >= (uint32_t)WAVE) {
+                gbase = g;
+                load_groups();
+            }
+            const int j = (int)(g - gbase);
+            const uint32_t key = rl(gk, j), hr = rl(ghr, j), g_end = rl(gend, j);
+            winding += (int32_t)rl((uint32_t)gsg, j);
+            if (!hr) {  // tile-row crossings only
+                e = g_end;
+                continue;
+            }
+            const int32_t tx = G.tx0 + (int32_t)(key & xmask);
+            if (have_prev && tx > prev_tx + 1 && prev_w != 0) {
+                if (lane == 0) {
+                    const uint32_t si = row_spans + ns;
+                    a.span_xy[2 * si] = (int16_t)((prev_tx + 1) * TILE);
+                    a.span_xy[2 * si + 1] = (int16_t)(ty * TILE);
+                    a.span_w[si] = (uint16_t)((uint16_t)(tx - prev_tx - 1) * (uint16_t)TILE);
+                }
+                ns++;
+            }
+            for (; e < g_end; e++) {
+                if (e - ebase >= (uint32_t)WAVE) {
+                    // (c) replay the next 64 entries, one run per lane
+                    ebase = e;
+                    const uint32_t ei = ebase + (uint32_t)lane;
+                    nrun = 0;
+                    if (ei < E_hi) {
+                        const uint32_t v = a.vals[ei];
+                        if (!(v >> 31)) {
+                            const RunRec r = a.recs[v];
+                            const float4 L = a.lines[r.line & 0x7fffffffu];
+                            Dda d;
+                            const V2 last = mk(L.x, L.y), point = mk(L.z, L.w);
+                            if (r.line >> 31) {
+                                dda_init(d, last, point);
+                            } else {
+                                dda_consts(d, last, point);
+                                d.x = (int16_t)(r.xy & 0xffffu);
+                                d.y = (int16_t)(r.xy >> 16);
+                                d.row_t1 = r.row_t1;
+                                d.col_t1 = r.col_t1;
+                                d.t0 = 0.0f;
+                                dda_step(d);
+                            }
+                            const int16_t rtx = div8(d.x), rty = div8(d.y);
+                            bool ended = false;
+                            for (int k = 0; k < TR_SLOTS; k++) {
+                                float area, height;
+                                dda_increment(d, area, height);
+                                inc[lane * TR_SLOTS + k] = make_float2(area, height);
+                                ipix[lane * TR_SLOTS + k] = (uint8_t)(((d.y & 7) << 3) | (d.x & 7));
+                                nrun = (uint32_t)k + 1;
+                                if (dda_step(d) || div8(d.x) != rtx || div8(d.y) != rty) {
+                                    ended = true;
+                                    break;
+                                }
+                            }
+                            if (!ended) atomicOr(&a.cnt->error, (uint32_t)ERR_REPLAY);
+                        }
+                    }
+                    __syncthreads();
+                }
+                const int jr = (int)(e - ebase);
+                const uint32_t n = rl(nrun, jr);
+                if ((uint32_t)lane < n) {
+                    const float2 v = inc[jr * TR_SLOTS + lane];
+                    const int px = ipix[jr * TR_SLOTS + lane];
+                    float2 c = acc[px];
+                    c.x = c.x + v.x;
+                    c.y = c.y + v.y;
+                    acc[px] = c;
+                }
+            }
+            // (d)(e) lane = pixel (y = lane >> 3, x = lane & 7)
+            const float2 c = acc[lane];
+            acc[lane] = make_float2(0.0f, 0.0f);
+            const int x = lane & 7;
+            float fold = carry, mine = 0.0f;
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const float hk = __shfl(c.y, (lane & ~7) | k, WAVE);
+                if (x == k) mine = fold;
+                fold = fold + hk;
+            }
+            carry = fold;
+            const float q = fminf(fabsf(mine + c.x) * 256.0f, 255.0f);
+            const uint32_t ti = row_tiles + nt;
+            a.tile_alpha[(size_t)ti * 64 + lane] = (uint8_t)(uint32_t)q;
+            if (lane == 0) {
+                a.tile_xy[2 * ti] = (int16_t)(tx * TILE);
+                a.tile_xy[2 * ti + 1] = (int16_t)(ty * TILE);
+            }
+            nt++;
+            have_prev = true;
+            prev_tx = tx;
+            prev_w = winding;
+        }
         __syncthreads();
     }
 }
@@ -959,8 +964,12 @@ static int run_plain_pass(Context* ctx, const SegInput& in, OutputDesc& out, hip
     da.xbits = 0;
     da.keys = da.vals = nullptr;
     da.recs = nullptr;
+    HIPCHK(ctx->dda_blk.ensure((size_t)(grid_for(std::max<int64_t>(L, 1), 256)) * 8));
+    da.blk_entries = ctx->dda_blk.as<uint64_t>();
     if (L > 0) {
         hipLaunchKernelGGL(k_dda<false>, dim3(grid_for(L, 256)), dim3(256), 0, s, da);
+        hipLaunchKernelGGL(k_sum_u64, dim3(std::min<uint32_t>(grid_for(grid_for(L, 256), 256), 256)), dim3(256), 0, s,
+                           da.blk_entries, (int64_t)grid_for(L, 256), &cnt->entries);
         HIPCHK(launch_scan<uint64_t>(ctx, L, LoadU64{da.line_cnt}, StoreU64{da.line_cnt}, OpAdd(), 0ull, &tot[2], s));
     }
     HIPCHK(hipGetLastError());
@@ -1080,7 +1089,7 @@ static int run_plain_pass(Context* ctx, const SegInput& in, OutputDesc& out, hip
     ta.span_w = out.span_w;
     ta.cnt = cnt;
     const uint32_t tile_blocks = (uint32_t)std::min<int64_t>((ROWS + ROWS_PER_WAVE - 1) / ROWS_PER_WAVE, 1 << 20);
-    hipLaunchKernelGGL(k_tiles<false>, dim3(tile_blocks), dim3(WAVE), 0, s, ta);
+    hipLaunchKernelGGL(k_tile_counts, dim3(tile_blocks), dim3(WAVE), 0, s, ta);
     HIPCHK(launch_scan<uint64_t>(ctx, ROWS, LoadU64{ta.row_cnt}, StoreU64{ctx->row_off.as<uint64_t>()}, OpAdd(),
                                  0ull, &tot[4], s));
     HIPCHK(hipGetLastError());
@@ -1100,7 +1109,7 @@ static int run_plain_pass(Context* ctx, const SegInput& in, OutputDesc& out, hip
         if (!ctx->ev[1]) HIPCHK(hipEventCreate(&ctx->ev[1]));
         HIPCHK(hipEventRecord(ctx->ev[0], s));
     }
-    hipLaunchKernelGGL(k_tiles<true>, dim3(tile_blocks), dim3(WAVE), 0, s, ta);
+    hipLaunchKernelGGL(k_tile_rows, dim3((uint32_t)std::min<int64_t>(ROWS, 1 << 20)), dim3(WAVE), 0, s, ta);
     if (ctx->timing) HIPCHK(hipEventRecord(ctx->ev[1], s));
     hipLaunchKernelGGL(k_path_out, dim3(grid_for(NP, 256)), dim3(256), 0, s, grid, NP, ta.row_off,
                        (const uint64_t*)&tot[4], out.path_tile_start, out.path_tile_count, out.path_span_start,
@@ -1110,7 +1119,7 @@ static int run_plain_pass(Context* ctx, const SegInput& in, OutputDesc& out, hip
         HIPCHK(hipEventSynchronize(ctx->ev[1]));
         float ms = 0;
         HIPCHK(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
-        auto& t = ctx->timings["k_tiles"];
+        auto& t = ctx->timings["k_tile_rows"];
         t.first += ms;
         t.second += 1;
     }
