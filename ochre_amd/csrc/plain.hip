@@ -750,7 +750,7 @@ __global__ __launch_bounds__(WAVE) void k_tile_rows(TileArgs a) {
                 }
                 ns++;
             }
-            for (; e < g_end; e++) {
+            while (e < g_end) {
                 if (e - ebase >= (uint32_t)WAVE) {
                     // (c) replay the next 64 entries, one run per lane
                     ebase = e;
@@ -792,16 +792,33 @@ __global__ __launch_bounds__(WAVE) void k_tile_rows(TileArgs a) {
                     }
                     __syncthreads();
                 }
-                const int jr = (int)(e - ebase);
-                const uint32_t n = rl(nrun, jr);
-                if ((uint32_t)lane < n) {
-                    const float2 v = inc[jr * TR_SLOTS + lane];
-                    const int px = ipix[jr * TR_SLOTS + lane];
-                    float2 c = acc[px];
-                    c.x = c.x + v.x;
-                    c.y = c.y + v.y;
-                    acc[px] = c;
+                // up to 4 runs of this block at a time: their increments are
+                // read first, then added in order (one LDS round trip per run)
+                const uint32_t jr = e - ebase;
+                const uint32_t m = min(min(4u, g_end - e), (uint32_t)WAVE - jr);
+                float2 v[4];
+                int px[4];
+                uint32_t nn[4];
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    nn[i] = (uint32_t)i < m ? rl(nrun, (int)jr + i) : 0u;
+                    v[i] = make_float2(0.0f, 0.0f);
+                    px[i] = 0;
+                    if ((uint32_t)lane < nn[i]) {
+                        v[i] = inc[(jr + i) * TR_SLOTS + lane];
+                        px[i] = ipix[(jr + i) * TR_SLOTS + lane];
+                    }
                 }
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    if ((uint32_t)lane < nn[i]) {
+                        float2 c = acc[px[i]];
+                        c.x = c.x + v[i].x;
+                        c.y = c.y + v[i].y;
+                        acc[px[i]] = c;
+                    }
+                }
+                e += m;
             }
             // (d)(e) lane = pixel (y = lane >> 3, x = lane & 7)
             const float2 c = acc[lane];
