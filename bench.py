@@ -155,6 +155,26 @@ def api_latency(reps=20):
             "what": "Rasterizer.fill + finish through the C ABI, TileBuilder callbacks included, median"}
 
 
+def plain_per_path(ctx, batch, n=200):
+    """The plain pipeline launched once per path (the drop-in use: one
+    Rasterizer per path, as examples/svg.rs:143-145 runs it), device-resident
+    input, on the first n paths."""
+    import torch
+    from ochre_amd.device import DeviceOutput, rasterize_plain, upload
+    n = min(n, batch.n_paths)
+    subs = [upload(_range_batch(batch, i, i + 1)) for i in range(n)]
+    out = DeviceOutput(1, 1 << 20, 1 << 16)
+    rasterize_plain(ctx, subs[0], out)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for sb in subs:
+        rasterize_plain(ctx, sb, out)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t
+    return {"paths_per_s": n / el, "ms_per_path": el / n * 1e3,
+            "sample": f"the first {n} paths of {batch.name}, one ochre_rasterize_plain call each"}
+
+
 def traffic_bytes(kernel, workload):
     """HBM bytes per launch of `kernel` on `workload` from the committed
     rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated), or
@@ -365,6 +385,13 @@ def main():
                      "kernel": res[head]["kernel"], "kernel_ms": kms, "algorithmic_bytes": alg},
     }
     line["rasterizer_api_latency"] = api_latency()
+    if not stroking:
+        try:
+            pp = plain_per_path(ctx, batch)
+            line["plain_per_path_launch"] = pp
+            line["batched_over_plain_per_path"] = paths_s / pp["paths_per_s"]
+        except Exception as e:  # a path too large for the probe's output: report without it
+            log(f"per-path plain probe skipped: {e}")
     if not args.no_cpu:
         try:
             line["cpu_baseline_all_cores"] = cpu_all_cores(batch)

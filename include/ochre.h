@@ -229,6 +229,12 @@ int ochre_context_get_dense_params(ochre_context* ctx, int* enabled, int* groups
  * faster; the autotuner tries both). */
 int ochre_context_set_dense_small(ochre_context* ctx, int groups_per_cu);
 int ochre_context_get_dense_small(ochre_context* ctx, int* groups_per_cu);
+/* A 256-tile dense instance (one workgroup of 16 waves and 130 KB of LDS per
+ * CU) behind the 64-tile one, over the paths it hands on (grids of 65-256
+ * tiles go there instead of to the ordered kernel); enabled 0 or 1, default
+ * 1.  The autotuner tries both. */
+int ochre_context_set_dense_large(ochre_context* ctx, int enabled);
+int ochre_context_get_dense_large(ochre_context* ctx, int* enabled);
 
 /* Per-pass limits of the plain pipeline: one pass holds at most max_lines
  * flattened lines and max_entries sort entries (defaults 2^29 and 2^30, about
@@ -265,8 +271,9 @@ int ochre_coverage_adjoint(ochre_context* ctx, const ochre_path_batch* in, const
 
 /* Autotuner: with the dense kernel on, times every (workgroups per CU
  * {1,2,3,4}, waves per workgroup {2,4}) of the 64-tile dense launch, then
- * the small-grid instance {off, 4, 6, 8 workgroups per CU}, then waves per
- * CU {1,2,4,8} of the ordered kernel behind them; with it off,
+ * the small-grid instance {off, 4, 6, 8 workgroups per CU}, the 256-tile
+ * instance {off, on}, then waves per CU {1,2,4,8} of the ordered kernel
+ * behind them; with it off,
  * every combination of waves per CU {2,3,4,6,8,12}, paths per pop {1,2,4,8}
  * and tiles per chunk {16,32} of the ordered launch.  Each configuration is
  * timed on the caller's batch (`reps` calls, median); the fastest is kept in
@@ -280,6 +287,7 @@ typedef struct ochre_batched_tuning {
     int dense_groups_per_cu; /* 0: the dense kernel was off */
     int dense_waves_per_group;
     int dense_small_groups_per_cu; /* the small-grid dense instance (0: off) */
+    int dense_large;               /* the 256-tile dense instance (0: off) */
 } ochre_batched_tuning;
 int ochre_autotune_batched(ochre_context* ctx, const ochre_path_batch* in, ochre_batch_output* out, void* stream,
                            int reps, ochre_batched_tuning* results, int max_results, int* n_results);

@@ -30,6 +30,7 @@ EXPORTS = [
     "ochre_rasterizer_stroke", "ochre_stroke_plain", "ochre_stroke_batched",
     "ochre_context_set_dense_params", "ochre_context_get_dense_params", "ochre_build_atlas",
     "ochre_context_set_dense_small", "ochre_context_get_dense_small", "ochre_context_set_pass_limits",
+    "ochre_context_set_dense_large", "ochre_context_get_dense_large",
 ]
 
 
@@ -71,7 +72,8 @@ class CoverageTargetC(ctypes.Structure):
 class BatchedTuningC(ctypes.Structure):
     _fields_ = [("waves_per_cu", ctypes.c_int), ("paths_per_pop", ctypes.c_int), ("tiles_per_chunk", ctypes.c_int),
                 ("ms", ctypes.c_float), ("dense_groups_per_cu", ctypes.c_int),
-                ("dense_waves_per_group", ctypes.c_int), ("dense_small_groups_per_cu", ctypes.c_int)]
+                ("dense_waves_per_group", ctypes.c_int), ("dense_small_groups_per_cu", ctypes.c_int),
+                ("dense_large", ctypes.c_int)]
 
 
 TILE_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int16, ctypes.c_int16, ctypes.POINTER(ctypes.c_uint8))
@@ -130,6 +132,9 @@ def _optional(lib):
         lib.ochre_context_get_dense_params.argtypes = [P] + [ctypes.POINTER(I)] * 3
         lib.ochre_context_set_dense_small.argtypes = [P, I]
         lib.ochre_context_get_dense_small.argtypes = [P, ctypes.POINTER(I)]
+    if hasattr(lib, "ochre_context_set_dense_large"):
+        lib.ochre_context_set_dense_large.argtypes = [P, ctypes.c_int]
+        lib.ochre_context_get_dense_large.argtypes = [P, ctypes.POINTER(ctypes.c_int)]
     if hasattr(lib, "ochre_context_set_pass_limits"):
         lib.ochre_context_set_pass_limits.argtypes = [P, ctypes.c_int64, ctypes.c_int64]
     if hasattr(lib, "ochre_build_atlas"):

@@ -80,8 +80,10 @@ struct FusedCtl {
     unsigned long long spans;
     uint32_t queue2;    // queue of k_fused over the handed-on paths
     uint32_t queue3;    // queue of the large-grid dense kernel
-    uint32_t dq2;       // paths handed on by the large-grid dense kernel
-    uint32_t pad[53];
+    uint32_t dq2;       // paths handed on by the second dense instance
+    uint32_t queue4;    // queue of the third dense instance
+    uint32_t dq3;       // paths handed on by the third dense instance
+    uint32_t pad[51];
 };
 
 __global__ void k_fused_reset(FusedCtl* c) {
@@ -95,6 +97,8 @@ __global__ void k_fused_reset(FusedCtl* c) {
         c->queue2 = 0;
         c->queue3 = 0;
         c->dq2 = 0;
+        c->queue4 = 0;
+        c->dq3 = 0;
     }
 }
 
@@ -788,7 +792,7 @@ __global__ void k_scatter_ranges(const uint32_t* ids, uint32_t n, const int64_t*
 }
 
 struct BatchedState {
-    DBuf scratch, ascratch, ctl, deferred, dscratch, dense_ids, dscratch2, dense_ids2;
+    DBuf scratch, ascratch, ctl, deferred, dscratch, dense_ids, dense_ids2, dense_ids3;
     DBuf g_verbs, g_points, g_weights, g_voff, g_poff, g_woff, g_xf, g_ts, g_tc, g_ss, g_sc;
     int n_waves = 0;
     uint32_t gran = 1;
@@ -850,26 +854,36 @@ int run_batched(Context* ctx, const SegInput& in, OutputDesc& out, hipStream_t s
     const int dnw = ctx->dense_waves == 2 ? 2 : 4;
     const int n_groups = std::max(1, cus * std::max(1, ctx->dense_groups_per_cu));
     const int n_small = dense ? cus * std::max(0, ctx->dense_small_groups) : 0;
-    uint8_t* dscratch = nullptr;
-    uint8_t* dscratch2 = nullptr;
-    uint32_t* ids_a = nullptr;
-    uint32_t* ids_b = nullptr;
-    FusedIn fi_a = fi, fi_b = fi;
+    const int n_large = (dense && ctx->dense_large) ? cus : 0;
+    // The dense instances run one after another, each over the paths the one
+    // before handed on: [small grids, 32 tiles] -> 64 tiles -> [256 tiles], then
+    // the ordered kernel over what is left.
+    enum { D_SMALL, D_MAIN, D_LARGE };
+    int stages[3], n_stages = 0;
     if (dense) {
-        HIPCHK_RET(B->dscratch.ensure((size_t)n_groups * D_SCRATCH));
+        if (n_small) stages[n_stages++] = D_SMALL;
+        stages[n_stages++] = D_MAIN;
+        if (n_large) stages[n_stages++] = D_LARGE;
+    }
+    uint8_t* dscratch = nullptr;
+    uint32_t* lists[3] = {nullptr, nullptr, nullptr};
+    uint32_t* counts[3] = {&ctl->dq, &ctl->dq2, &ctl->dq3};
+    uint32_t* queues[3] = {&ctl->queue, &ctl->queue3, &ctl->queue4};
+    FusedIn fin[4] = {fi, fi, fi, fi};  // stage i reads fin[i]; k_fused reads fin[n_stages]
+    if (dense) {
+        const int most = std::max({n_groups, n_small, n_large});
+        HIPCHK_RET(B->dscratch.ensure((size_t)most * D_SCRATCH));  // the stages run one at a time
         HIPCHK_RET(B->dense_ids.ensure((size_t)NP * 4));
         HIPCHK_RET(B->dense_ids2.ensure((size_t)NP * 4));
-        if (n_small) HIPCHK_RET(B->dscratch2.ensure((size_t)n_small * D_SCRATCH));
+        if (n_stages > 2) HIPCHK_RET(B->dense_ids3.ensure((size_t)NP * 4));
         dscratch = B->dscratch.as<uint8_t>();
-        dscratch2 = B->dscratch2.as<uint8_t>();
-        ids_a = B->dense_ids.as<uint32_t>();
-        ids_b = B->dense_ids2.as<uint32_t>();
-        // small grids first (all paths -> list a), then large grids (list a ->
-        // list b), then the ordered kernel (list b)
-        fi_a.ids = ids_a;
-        fi_a.n_ids = &ctl->dq;
-        fi_b.ids = ids_b;
-        fi_b.n_ids = &ctl->dq2;
+        lists[0] = B->dense_ids.as<uint32_t>();
+        lists[1] = B->dense_ids2.as<uint32_t>();
+        lists[2] = B->dense_ids3.as<uint32_t>();
+        for (int i = 1; i <= n_stages; i++) {
+            fin[i].ids = lists[i - 1];
+            fin[i].n_ids = counts[i - 1];
+        }
     }
     const bool timing = ctx->timing;
     if (timing) {
@@ -880,23 +894,23 @@ int run_batched(Context* ctx, const SegInput& in, OutputDesc& out, hipStream_t s
     auto enqueue = [&](hipStream_t st, bool ev) {
         hipLaunchKernelGGL(k_fused_reset, dim3(1), dim3(64), 0, st, ctl);
         if (ev) (void)hipEventRecord(ctx->ev[0], st);
-        if (dense) {
-            const FusedIn& f_large = n_small ? fi_a : fi;
-            if (n_small)
-                hipLaunchKernelGGL((k_dense<2, 32>), dim3(n_small), dim3(2 * WAVE), 0, st, fi, fo, ctl, &ctl->queue,
-                                   ids_a, &ctl->dq, dscratch2);
-            uint32_t* out_ids = n_small ? ids_b : ids_a;
-            uint32_t* out_cnt = n_small ? &ctl->dq2 : &ctl->dq;
-            uint32_t* q = n_small ? &ctl->queue3 : &ctl->queue;
-            if (dnw == 2)
-                hipLaunchKernelGGL((k_dense<2, 64>), dim3(n_groups), dim3(2 * WAVE), 0, st, f_large, fo, ctl, q,
-                                   out_ids, out_cnt, dscratch);
+        for (int i = 0; i < n_stages; i++) {
+            const FusedIn& f = fin[i];
+            if (stages[i] == D_SMALL)
+                hipLaunchKernelGGL((k_dense<2, 32>), dim3(n_small), dim3(2 * WAVE), 0, st, f, fo, ctl, queues[i],
+                                   lists[i], counts[i], dscratch);
+            else if (stages[i] == D_LARGE)
+                hipLaunchKernelGGL((k_dense<16, 256>), dim3(n_large), dim3(16 * WAVE), 0, st, f, fo, ctl, queues[i],
+                                   lists[i], counts[i], dscratch);
+            else if (dnw == 2)
+                hipLaunchKernelGGL((k_dense<2, 64>), dim3(n_groups), dim3(2 * WAVE), 0, st, f, fo, ctl, queues[i],
+                                   lists[i], counts[i], dscratch);
             else
-                hipLaunchKernelGGL((k_dense<4, 64>), dim3(n_groups), dim3(4 * WAVE), 0, st, f_large, fo, ctl, q,
-                                   out_ids, out_cnt, dscratch);
-            if (ev) (void)hipEventRecord(ctx->ev[2], st);
+                hipLaunchKernelGGL((k_dense<4, 64>), dim3(n_groups), dim3(4 * WAVE), 0, st, f, fo, ctl, queues[i],
+                                   lists[i], counts[i], dscratch);
         }
-        const FusedIn& f = dense ? (n_small ? fi_b : fi_a) : fi;
+        if (dense && ev) (void)hipEventRecord(ctx->ev[2], st);
+        const FusedIn& f = fin[n_stages];
         if (tc16) hipLaunchKernelGGL(k_fused<16>, dim3(n_waves), dim3(WAVE), 0, st, f, fo, ctl, deferred, scratch, g);
         else hipLaunchKernelGGL(k_fused<32>, dim3(n_waves), dim3(WAVE), 0, st, f, fo, ctl, deferred, scratch, g);
         if (ev) (void)hipEventRecord(ctx->ev[1], st);
@@ -913,6 +927,7 @@ int run_batched(Context* ctx, const SegInput& in, OutputDesc& out, hipStream_t s
         const int dkey = dense ? (dnw << 16) | n_groups : 0;
         memcpy(key.data() + sizeof(fi) + sizeof(fo) + 24, &dkey, 4);
         memcpy(key.data() + sizeof(fi) + sizeof(fo) + 28, &n_small, 4);
+        memcpy(key.data() + sizeof(fi) + sizeof(fo) + 32, &n_large, 4);
         if (!B->exec || key != B->graph_key) {
             if (B->exec) (void)hipGraphExecDestroy(B->exec);
             if (B->graph) (void)hipGraphDestroy(B->graph);

@@ -272,6 +272,20 @@ int ochre_context_set_pass_limits(ochre_context* ctx, int64_t max_lines, int64_t
     return OCHRE_OK;
 }
 
+int ochre_context_set_dense_large(ochre_context* ctx, int enabled) {
+    if (!ctx || enabled < 0 || enabled > 1) return OCHRE_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->c.dense_large = enabled;
+    return OCHRE_OK;
+}
+
+int ochre_context_get_dense_large(ochre_context* ctx, int* enabled) {
+    if (!ctx || !enabled) return OCHRE_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    *enabled = ctx->c.dense_large;
+    return OCHRE_OK;
+}
+
 int ochre_context_get_dense_small(ochre_context* ctx, int* groups_per_cu) {
     if (!ctx || !groups_per_cu) return OCHRE_ERR_INVALID_ARG;
     std::lock_guard<std::mutex> lk(ctx->mu);
@@ -325,7 +339,7 @@ int ochre_autotune_batched(ochre_context* ctx, const ochre_path_batch* in, ochre
     static const int tcs[] = {16, 32};
     static const int dgroups[] = {1, 2, 3, 4};
     static const int dwaves[] = {2, 4};
-    ochre_batched_tuning best{ctx->c.batched_waves_per_cu, ctx->c.batched_gran, ctx->c.batched_tc, 1e30f, 0, 0, 0};
+    ochre_batched_tuning best{ctx->c.batched_waves_per_cu, ctx->c.batched_gran, ctx->c.batched_tc, 1e30f, 0, 0, 0, 0};
     int best_dg = ctx->c.dense_groups_per_cu, best_dw = ctx->c.dense_waves;
     int k = 0;
     auto measure = [&](int w, int g, int tc, float* ms_out) -> int {
@@ -357,7 +371,7 @@ int ochre_autotune_batched(ochre_context* ctx, const ochre_path_batch* in, ochre
                 if (st) return st;
                 if (results && k < max_results)
                     results[k] = ochre_batched_tuning{best.waves_per_cu, best.paths_per_pop, best.tiles_per_chunk, ms,
-                                                      dg, dw, ctx->c.dense_small_groups};
+                                                      dg, dw, ctx->c.dense_small_groups, ctx->c.dense_large};
                 k++;
                 if (ms < best.ms) {
                     best.ms = ms;
@@ -376,7 +390,7 @@ int ochre_autotune_batched(ochre_context* ctx, const ochre_path_batch* in, ochre
             if (st) return st;
             if (results && k < max_results)
                 results[k] = ochre_batched_tuning{best.waves_per_cu, best.paths_per_pop, best.tiles_per_chunk, ms,
-                                                  best_dg, best_dw, sg};
+                                                  best_dg, best_dw, sg, ctx->c.dense_large};
             k++;
             if (ms < best.ms) {
                 best.ms = ms;
@@ -384,6 +398,23 @@ int ochre_autotune_batched(ochre_context* ctx, const ochre_path_batch* in, ochre
             }
         }
         ctx->c.dense_small_groups = best_small;
+        // the 256-tile instance behind: off or on
+        int best_large = ctx->c.dense_large;
+        for (int lg : {0, 1}) {
+            ctx->c.dense_large = lg;
+            float ms = 0;
+            st = measure(best.waves_per_cu, best.paths_per_pop, best.tiles_per_chunk, &ms);
+            if (st) return st;
+            if (results && k < max_results)
+                results[k] = ochre_batched_tuning{best.waves_per_cu, best.paths_per_pop, best.tiles_per_chunk, ms,
+                                                  best_dg, best_dw, best_small, lg};
+            k++;
+            if (ms < best.ms) {
+                best.ms = ms;
+                best_large = lg;
+            }
+        }
+        ctx->c.dense_large = best_large;
         // then the persistent grid of the ordered kernel behind them, which
         // only works through the paths the dense kernel hands on
         for (int w : {1, 2, 4, 8}) {
@@ -392,7 +423,7 @@ int ochre_autotune_batched(ochre_context* ctx, const ochre_path_batch* in, ochre
             if (st) return st;
             if (results && k < max_results)
                 results[k] = ochre_batched_tuning{w, best.paths_per_pop, best.tiles_per_chunk, ms, best_dg, best_dw,
-                                                  ctx->c.dense_small_groups};
+                                                  ctx->c.dense_small_groups, ctx->c.dense_large};
             k++;
             if (ms < best.ms) {
                 best.ms = ms;
@@ -406,9 +437,9 @@ int ochre_autotune_batched(ochre_context* ctx, const ochre_path_batch* in, ochre
                     float ms = 0;
                     st = measure(w, g, tc, &ms);
                     if (st) return st;
-                    if (results && k < max_results) results[k] = ochre_batched_tuning{w, g, tc, ms, 0, 0, 0};
+                    if (results && k < max_results) results[k] = ochre_batched_tuning{w, g, tc, ms, 0, 0, 0, 0};
                     k++;
-                    if (ms < best.ms) best = ochre_batched_tuning{w, g, tc, ms, 0, 0, 0};
+                    if (ms < best.ms) best = ochre_batched_tuning{w, g, tc, ms, 0, 0, 0, 0};
                 }
     }
     if (n_results) *n_results = k;

@@ -148,6 +148,7 @@ struct Context {
     int dense_groups_per_cu = 4;
     int dense_waves = 4;
     int dense_small_groups = 0;  // small-grid instance in front (0: off)
+    int dense_large = 1;         // 256-tile instance behind (1 workgroup of 16 waves per CU; 0: off)
     StrokeBuffers stroke;
     AtlasState atlas;
     ~Context();

@@ -677,27 +677,28 @@ __global__ __launch_bounds__(WAVE) void k_tile_counts(TileArgs a) {
 
 // Emit pass of the tile stage: one wave per tile row, walking the row's
 // groups (bins, rasterizer.rs:213-265) in key order.  The row's sort entries
-// are taken 64 at a time and each lane replays one run into LDS (a run stays
-// inside one 8x8 tile and the DDA never revisits a pixel, so it has at most
-// 15 increments, all at distinct pixels).  The runs are then folded in sorted
-// order, one run per step, lane k adding the run's increment k to its pixel's
-// sums: a wave's LDS operations complete in issue order, so every pixel's f32
-// sums follow the sorted order exactly (rasterizer.rs:221-228).  At the end of
-// a tile, lane = pixel: the row carry is folded across the 8 lanes of each
-// pixel row in x order, the alphas quantised and stored (rasterizer.rs:230-250),
-// and a span is written where the previous tile of the row is more than one
-// tile to the left and the winding after it is non-zero (rasterizer.rs:252-264).
+// are taken TR_BLOCK at a time and each of the first TR_BLOCK lanes replays
+// one run into LDS: its increments in step order, and a 64-byte map from
+// pixel to step (a run stays inside one 8x8 tile and the DDA never revisits a
+// pixel, so a run has at most 15 increments, at distinct pixels).  Lane p then
+// owns pixel p of the tile: for each run in sorted order it looks its pixel up
+// in the run's map and adds the increment found there to its register sums,
+// so every pixel's f32 sums follow the sorted order exactly
+// (rasterizer.rs:221-228), with no read-modify-write through LDS.  At the end
+// of a tile the row carry is folded across the 8 lanes of each pixel row in x
+// order, the alphas quantised and stored (rasterizer.rs:230-250), and a span
+// is written where the previous tile of the row is more than one tile to the
+// left and the winding after it is non-zero (rasterizer.rs:252-264).
 constexpr int TR_SLOTS = 16;  // increments per run slot
+constexpr int TR_BLOCK = 32;  // runs replayed per block (6 KB of LDS per wave)
 
 __device__ inline uint32_t rl(uint32_t v, int j) { return (uint32_t)__builtin_amdgcn_readlane((int)v, j); }
 
 __global__ __launch_bounds__(WAVE) void k_tile_rows(TileArgs a) {
-    __shared__ float2 inc[WAVE * TR_SLOTS];
-    __shared__ uint8_t ipix[WAVE * TR_SLOTS];
-    __shared__ float2 acc[64];
+    __shared__ float2 inc[TR_BLOCK * TR_SLOTS];
+    __shared__ __align__(16) uint8_t smap[TR_BLOCK * 64];  // run j, pixel p -> step + 1 (0: not visited)
     const int lane = threadIdx.x;
     const uint32_t xmask = (1u << a.xbits) - 1;
-    acc[lane] = make_float2(0.0f, 0.0f);
     for (int64_t row = blockIdx.x; row < a.n_rows; row += gridDim.x) {
         const PathGrid G = a.grid[a.row_path[row]];
         if (row - (int64_t)G.row_base >= G.h) continue;  // the empty-path row (k_path_out)
@@ -726,8 +727,8 @@ __global__ __launch_bounds__(WAVE) void k_tile_rows(TileArgs a) {
         };
         load_groups();
         uint32_t e = a.gbegin[gb];
-        uint32_t ebase = e + WAVE;  // no entry block loaded yet
-        uint32_t nrun = 0;          // increments of the run this lane replayed
+        uint32_t ebase = e + TR_BLOCK;  // no entry block loaded yet
+        float pa = 0.0f, ph = 0.0f;     // this lane's pixel sums for the current tile
         for (uint32_t g = gb; g < ge; g++) {
             if (g - gbase >= (uint32_t)WAVE) {
                 gbase = g;
@@ -751,13 +752,16 @@ __global__ __launch_bounds__(WAVE) void k_tile_rows(TileArgs a) {
                 ns++;
             }
             while (e < g_end) {
-                if (e - ebase >= (uint32_t)WAVE) {
-                    // (c) replay the next 64 entries, one run per lane
+                if (e - ebase >= (uint32_t)TR_BLOCK) {
+                    // (c) replay the next TR_BLOCK entries, one run per lane
                     ebase = e;
-                    const uint32_t ei = ebase + (uint32_t)lane;
-                    nrun = 0;
-                    if (ei < E_hi) {
-                        const uint32_t v = a.vals[ei];
+                    __syncthreads();  // the previous block's maps are read
+                    if (lane < TR_BLOCK) {
+                        uint4* row_map = reinterpret_cast<uint4*>(smap + lane * 64);
+#pragma unroll
+                        for (int k = 0; k < 4; k++) row_map[k] = make_uint4(0, 0, 0, 0);
+                        const uint32_t ei = ebase + (uint32_t)lane;
+                        const uint32_t v = ei < E_hi ? a.vals[ei] : 0x80000000u;
                         if (!(v >> 31)) {
                             const RunRec r = a.recs[v];
                             const float4 L = a.lines[r.line & 0x7fffffffu];
@@ -780,8 +784,7 @@ __global__ __launch_bounds__(WAVE) void k_tile_rows(TileArgs a) {
                                 float area, height;
                                 dda_increment(d, area, height);
                                 inc[lane * TR_SLOTS + k] = make_float2(area, height);
-                                ipix[lane * TR_SLOTS + k] = (uint8_t)(((d.y & 7) << 3) | (d.x & 7));
-                                nrun = (uint32_t)k + 1;
+                                smap[lane * 64 + (((d.y & 7) << 3) | (d.x & 7))] = (uint8_t)(k + 1);
                                 if (dda_step(d) || div8(d.x) != rtx || div8(d.y) != rty) {
                                     ended = true;
                                     break;
@@ -792,47 +795,35 @@ __global__ __launch_bounds__(WAVE) void k_tile_rows(TileArgs a) {
                     }
                     __syncthreads();
                 }
-                // up to 4 runs of this block at a time: their increments are
-                // read first, then added in order (one LDS round trip per run)
+                // up to 8 runs of this block at a time, added in sorted order
                 const uint32_t jr = e - ebase;
-                const uint32_t m = min(min(4u, g_end - e), (uint32_t)WAVE - jr);
-                float2 v[4];
-                int px[4];
-                uint32_t nn[4];
+                const uint32_t m = min(min(8u, g_end - e), (uint32_t)TR_BLOCK - jr);
+                uint32_t k8[8];
 #pragma unroll
-                for (int i = 0; i < 4; i++) {
-                    nn[i] = (uint32_t)i < m ? rl(nrun, (int)jr + i) : 0u;
-                    v[i] = make_float2(0.0f, 0.0f);
-                    px[i] = 0;
-                    if ((uint32_t)lane < nn[i]) {
-                        v[i] = inc[(jr + i) * TR_SLOTS + lane];
-                        px[i] = ipix[(jr + i) * TR_SLOTS + lane];
-                    }
-                }
+                for (int i = 0; i < 8; i++) k8[i] = (uint32_t)i < m ? smap[(jr + i) * 64 + lane] : 0u;
 #pragma unroll
-                for (int i = 0; i < 4; i++) {
-                    if ((uint32_t)lane < nn[i]) {
-                        float2 c = acc[px[i]];
-                        c.x = c.x + v[i].x;
-                        c.y = c.y + v[i].y;
-                        acc[px[i]] = c;
+                for (int i = 0; i < 8; i++) {
+                    if (k8[i]) {
+                        const float2 v = inc[(jr + i) * TR_SLOTS + k8[i] - 1];
+                        pa = pa + v.x;
+                        ph = ph + v.y;
                     }
                 }
                 e += m;
             }
             // (d)(e) lane = pixel (y = lane >> 3, x = lane & 7)
-            const float2 c = acc[lane];
-            acc[lane] = make_float2(0.0f, 0.0f);
             const int x = lane & 7;
             float fold = carry, mine = 0.0f;
 #pragma unroll
             for (int k = 0; k < 8; k++) {
-                const float hk = __shfl(c.y, (lane & ~7) | k, WAVE);
+                const float hk = __shfl(ph, (lane & ~7) | k, WAVE);
                 if (x == k) mine = fold;
                 fold = fold + hk;
             }
             carry = fold;
-            const float q = fminf(fabsf(mine + c.x) * 256.0f, 255.0f);
+            const float q = fminf(fabsf(mine + pa) * 256.0f, 255.0f);
+            pa = 0.0f;
+            ph = 0.0f;
             const uint32_t ti = row_tiles + nt;
             a.tile_alpha[(size_t)ti * 64 + lane] = (uint8_t)(uint32_t)q;
             if (lane == 0) {
@@ -844,7 +835,6 @@ __global__ __launch_bounds__(WAVE) void k_tile_rows(TileArgs a) {
             prev_tx = tx;
             prev_w = winding;
         }
-        __syncthreads();
     }
 }
 

@@ -268,16 +268,22 @@ def dense_params(ctx: Context):
     en, gpc, wpg = (ctypes.c_int() for _ in range(3))
     _lib.check(ctx.lib.ochre_context_get_dense_params(ctx.h, ctypes.byref(en), ctypes.byref(gpc), ctypes.byref(wpg)),
                "get_dense_params")
-    sg = ctypes.c_int()
+    sg, lg = ctypes.c_int(), ctypes.c_int()
     _lib.check(ctx.lib.ochre_context_get_dense_small(ctx.h, ctypes.byref(sg)), "get_dense_small")
+    _lib.check(ctx.lib.ochre_context_get_dense_large(ctx.h, ctypes.byref(lg)), "get_dense_large")
     return {"dense": en.value, "dense_groups_per_cu": gpc.value, "dense_waves_per_group": wpg.value,
-            "dense_small_groups_per_cu": sg.value}
+            "dense_small_groups_per_cu": sg.value, "dense_large": lg.value}
 
 
 def set_pass_limits(ctx: Context, max_lines=0, max_entries=0):
     """Per-pass limits of the plain pipeline (0: default); a batch over them
     is rasterised as consecutive path ranges."""
     _lib.check(ctx.lib.ochre_context_set_pass_limits(ctx.h, int(max_lines), int(max_entries)), "set_pass_limits")
+
+
+def set_dense_large(ctx: Context, enabled=True):
+    """The 256-tile dense instance behind the 64-tile one."""
+    _lib.check(ctx.lib.ochre_context_set_dense_large(ctx.h, 1 if enabled else 0), "set_dense_large")
 
 
 def set_dense_small(ctx: Context, groups_per_cu=0):
@@ -315,6 +321,7 @@ def autotune(ctx: Context, db: DeviceBatch, out: DeviceOutput, reps=3, cache_key
                 set_dense_params(ctx, bool(cached["dense"]), cached["dense_groups_per_cu"],
                                  cached["dense_waves_per_group"])
                 set_dense_small(ctx, cached.get("dense_small_groups_per_cu", 0))
+                set_dense_large(ctx, bool(cached.get("dense_large", 1)))
             return cached, []
     res = (_lib.BatchedTuningC * 64)()
     n = ctypes.c_int()
@@ -323,7 +330,7 @@ def autotune(ctx: Context, db: DeviceBatch, out: DeviceOutput, reps=3, cache_key
                                               res, 64, ctypes.byref(n)), "ochre_autotune_batched")
     allm = [{"waves_per_cu": r.waves_per_cu, "paths_per_pop": r.paths_per_pop, "tiles_per_chunk": r.tiles_per_chunk,
              "dense_groups_per_cu": r.dense_groups_per_cu, "dense_waves_per_group": r.dense_waves_per_group,
-             "dense_small_groups_per_cu": r.dense_small_groups_per_cu,
+             "dense_small_groups_per_cu": r.dense_small_groups_per_cu, "dense_large": r.dense_large,
              "ms": r.ms} for r in res[:n.value]]
     best = dict(batched_params(ctx))
     best["ms"] = min(m["ms"] for m in allm) if allm else None

@@ -226,21 +226,24 @@ def test_batched_params_and_graph_replays(gpu_ctx):
     ref = oracle_ref.run_batch(batch)
     db = upload(batch)
     lib = gpu_ctx.lib
-    # (dense, groups/CU, waves/group, small-grid groups/CU, waves/CU, paths/pop, tiles/chunk, graph)
-    configs = [(0, 0, 0, 0, 2, 1, 32, 1), (0, 0, 0, 0, 6, 1, 16, 1), (0, 0, 0, 0, 8, 4, 32, 1),
-               (0, 0, 0, 0, 4, 16, 16, 0), (0, 0, 0, 0, 1, 2, 32, 0), (1, 4, 4, 8, 6, 2, 32, 1),
-               (1, 2, 2, 0, 6, 2, 32, 1), (1, 1, 4, 4, 2, 1, 16, 0), (1, 3, 4, 6, 8, 1, 32, 1), (1, 4, 4, 0, 1, 1, 32, 1)]
+    # (dense, groups/CU, waves/group, small-grid groups/CU, 256-tile instance, waves/CU, paths/pop,
+    #  tiles/chunk, graph)
+    configs = [(0, 0, 0, 0, 0, 2, 1, 32, 1), (0, 0, 0, 0, 0, 6, 1, 16, 1), (0, 0, 0, 0, 0, 8, 4, 32, 1),
+               (0, 0, 0, 0, 0, 4, 16, 16, 0), (0, 0, 0, 0, 0, 1, 2, 32, 0), (1, 4, 4, 8, 1, 6, 2, 32, 1),
+               (1, 2, 2, 0, 0, 6, 2, 32, 1), (1, 1, 4, 4, 1, 2, 1, 16, 0), (1, 3, 4, 6, 0, 8, 1, 32, 1),
+               (1, 4, 4, 0, 1, 1, 1, 32, 1), (1, 4, 4, 0, 0, 1, 2, 32, 1)]
     try:
-        for dense, dg, dw, sg, waves, gran, tc, graph in configs:
+        for dense, dg, dw, sg, lg, waves, gran, tc, graph in configs:
             assert lib.ochre_context_set_dense_params(gpu_ctx.h, dense, max(dg, 1), dw or 4) == 0
             assert lib.ochre_context_set_dense_small(gpu_ctx.h, sg) == 0
+            assert lib.ochre_context_set_dense_large(gpu_ctx.h, lg) == 0
             assert lib.ochre_context_set_batched_params(gpu_ctx.h, waves, gran, tc, graph) == 0
             for rep in range(3):
                 out = DeviceOutput(batch.n_paths, ref.n_tiles + 16, ref.n_spans + 16)
                 rasterize_batched(gpu_ctx, db, out)
                 torch.cuda.synchronize()
                 st = compare(ref, out.to_host(), max_lsb=1 if dense else 0)
-                print(f"[batched] dense={dense}/{dg}/{dw}/{sg} waves={waves} gran={gran} tc={tc} graph={graph} "
+                print(f"[batched] dense={dense}/{dg}/{dw}/{sg}/{lg} waves={waves} gran={gran} tc={tc} graph={graph} "
                       f"rep={rep}: {st}", flush=True)
                 if dense:
                     assert st["bit_exact_fraction"] >= 0.999
@@ -249,3 +252,4 @@ def test_batched_params_and_graph_replays(gpu_ctx):
         lib.ochre_context_set_batched_params(gpu_ctx.h, 6, 2, 32, 1)
         lib.ochre_context_set_dense_params(gpu_ctx.h, 1, 4, 4)
         lib.ochre_context_set_dense_small(gpu_ctx.h, 0)
+        lib.ochre_context_set_dense_large(gpu_ctx.h, 1)
