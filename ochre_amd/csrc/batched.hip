@@ -55,7 +55,13 @@ struct FusedIn {
     // k_dense) instead of 0 .. n_paths
     const uint32_t* ids = nullptr;
     const uint32_t* n_ids = nullptr;
+    // more handed-on paths than this go to the plain pipeline whole: one wave
+    // per path is slow on the large paths the dense kernels hand on, and the
+    // plain pipeline's fixed cost is repaid from a few hundred of them
+    uint32_t defer_above = 0xffffffffu;
 };
+
+constexpr uint32_t F_DEFER_ABOVE = 512;  // handed-on paths beyond which k_fused defers them all
 
 struct FusedOut {
     int16_t* tile_xy;
@@ -746,6 +752,11 @@ __global__ __launch_bounds__(WAVE) void k_fused(FusedIn in, FusedOut out, FusedC
     __shared__ LdsState<TC> s;
     uint8_t* sc = scratch + (size_t)blockIdx.x * F_SCRATCH;
     const int64_t n = in.ids ? (int64_t)*in.n_ids : in.n_paths;
+    if (in.ids && n > (int64_t)in.defer_above) {
+        for (int64_t q = (int64_t)blockIdx.x * WAVE + threadIdx.x; q < n; q += (int64_t)gridDim.x * WAVE)
+            deferred[atomicAdd(&ctl->deferred, 1u)] = in.ids[q];
+        return;
+    }
     for (;;) {
         uint32_t base = 0;
         if (threadIdx.x == 0) base = atomicAdd(in.ids ? &ctl->queue2 : &ctl->queue, gran);
@@ -884,6 +895,7 @@ int run_batched(Context* ctx, const SegInput& in, OutputDesc& out, hipStream_t s
             fin[i].ids = lists[i - 1];
             fin[i].n_ids = counts[i - 1];
         }
+        fin[n_stages].defer_above = F_DEFER_ABOVE;
     }
     const bool timing = ctx->timing;
     if (timing) {
@@ -1045,7 +1057,9 @@ int run_batched(Context* ctx, const SegInput& in, OutputDesc& out, hipStream_t s
         od.path_tile_count = B->g_tc.as<int32_t>();
         od.path_span_start = B->g_ss.as<int64_t>();
         od.path_span_count = B->g_sc.as<int32_t>();
-        st = run_plain(ctx, si, od, s);
+        // under the dense kernels' 1-LSB contract the handed-on paths take the
+        // unordered tile stage; with them off the plain pipeline stays exact
+        st = run_plain(ctx, si, od, s, dense);
         Tp = od.n_tiles;
         Sp = od.n_spans;
         if (st == 0) {

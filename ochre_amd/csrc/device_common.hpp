@@ -198,6 +198,9 @@ __device__ inline void dda_init(Dda& d, V2 last, V2 point) {
     }
 }
 
+// The line's point at parameter t: (1 - t) * last + t * point (rasterizer.rs:100-101).
+__device__ inline V2 dda_point(const Dda& d, float t) { return add(smul(1.0f - t, d.last), smul(t, d.point)); }
+
 // The increment pushed at the current pixel (rasterizer.rs:98-106).
 __device__ inline void dda_increment(const Dda& d, float& area, float& height) {
     float t1 = fminf(d.row_t1, d.col_t1);

@@ -128,6 +128,7 @@ struct Context {
     DBuf grp_begin, grp_key, grp_hasrun, grp_sign;
     DBuf row_gbegin, row_gend, row_sign, row_wb, row_path, row_cnt, row_off;
     DBuf split_off;  // rebased per-path offsets of a path range (run_plain's split)
+    DBuf u_pwbase, u_rwbase, u_bcnt, u_bbeg, u_bcur;  // unordered tile stage: window buckets
     double split_ratio = 2.0;  // how far the last pass that had to split was over its limits
     int64_t max_lines = PLAIN_MAX_LINES, max_entries = PLAIN_MAX_ENTRIES;  // per-pass limits (lowered by tests)
     // host staging
@@ -155,7 +156,10 @@ struct Context {
 };
 
 // The plain five-kernel pipeline; see the per-pass limits above.
-int run_plain(Context* ctx, const SegInput& in, OutputDesc& out, hipStream_t stream);
+// `unordered` (used for the paths the batched pipeline hands on, under the
+// dense kernels' 1-LSB contract) replaces the sort and the ordered tile pass
+// by window buckets and LDS float-atomic sums.
+int run_plain(Context* ctx, const SegInput& in, OutputDesc& out, hipStream_t stream, bool unordered = false);
 // Persistent work-queue pipeline (one wave per path); paths that do not fit a
 // wave are deferred to run_plain.  waves_per_cu sets the persistent grid,
 // gran the paths taken per queue pop; use_graph replays (reset, kernel) from

@@ -838,6 +838,318 @@ __global__ __launch_bounds__(WAVE) void k_tile_rows(TileArgs a) {
     }
 }
 
+// ------------------------------------------- unordered tile stage (batched)
+// For the large paths the batched pipeline hands on: instead of the radix
+// sort, the sort entries are bucketed by (tile row, window of UW tiles) with
+// no order inside a bucket, and each window's runs are replayed into LDS sums
+// with float atomics, in arrival order — alpha within 1 LSB of the ordered
+// result where a sum sits on a quantisation boundary, as in the dense kernels;
+// the tile and span sets, which are integer, are exact.  One workgroup per tile
+// row walks the row's windows left to right, carrying the winding, the last
+// tile and the per-pixel-row height carry across them.
+constexpr int UW = 64;  // tiles per window
+
+struct LoadNWin {
+    const PathGrid* g;
+    __device__ uint32_t operator()(int64_t p) const { return (uint32_t)g[p].h * (((uint32_t)g[p].w + UW - 1) / UW); }
+};
+
+// row -> path and the row's first window bucket
+__global__ void k_rows_setup(const PathGrid* grid, int64_t n_paths, const uint32_t* pwbase, uint32_t* row_path,
+                             uint32_t* row_wbase) {
+    const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (p >= n_paths) return;
+    const PathGrid g = grid[p];
+    const uint32_t nwin = ((uint32_t)g.w + UW - 1) / UW;
+    for (int r = 0; r <= g.h; r++) {  // the last one is the empty-path row
+        row_path[g.row_base + r] = (uint32_t)p;
+        row_wbase[g.row_base + r] = pwbase[p] + (uint32_t)r * nwin;
+    }
+}
+
+__device__ inline uint32_t entry_bucket(uint32_t key, uint32_t xbits, const uint32_t* row_wbase) {
+    return row_wbase[key >> xbits] + ((key & ((1u << xbits) - 1u)) / UW);
+}
+
+// Bucket sizes (one atomic per run of equal buckets in a wave) and the
+// crossing sum of every tile row.
+__global__ __launch_bounds__(256) void k_bucket_hist(const uint32_t* keys, const uint32_t* vals, int64_t R,
+                                                     uint32_t xbits, const uint32_t* row_wbase, uint32_t* bcnt,
+                                                     int32_t* row_sign) {
+    const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const bool valid = e < R;
+    uint32_t b = 0xffffffffu;
+    if (valid) {
+        const uint32_t key = keys[e], v = vals[e];
+        b = entry_bucket(key, xbits, row_wbase);
+        if (v >> 31) {
+            const int sg = (int)(int8_t)(v & 0xffu);
+            if (sg) atomicAdd(&row_sign[key >> xbits], sg);
+        }
+    }
+    const int lane = threadIdx.x & (WAVE - 1);
+    const uint32_t prev = __shfl_up(b, 1, WAVE), next = __shfl_down(b, 1, WAVE);
+    const bool head = lane == 0 || prev != b;
+    const bool tail = lane == WAVE - 1 || next != b;
+    const uint64_t heads = __ballot(head);
+    if (valid && tail) {
+        const uint64_t below = heads & ((2ull << lane) - 1);  // my segment's head: highest head <= lane
+        const int h = 63 - __clzll((long long)below);
+        atomicAdd(&bcnt[b], (uint32_t)(lane - h + 1));
+    }
+}
+
+// Scatter into the buckets (one cursor atomic per run of equal buckets).
+__global__ __launch_bounds__(256) void k_bucket_scatter(const uint32_t* keys, const uint32_t* vals, int64_t R,
+                                                        uint32_t xbits, const uint32_t* row_wbase, uint32_t* bcur,
+                                                        uint32_t* okeys, uint32_t* ovals) {
+    const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const bool valid = e < R;
+    uint32_t b = 0xffffffffu, key = 0, v = 0;
+    if (valid) {
+        key = keys[e];
+        v = vals[e];
+        b = entry_bucket(key, xbits, row_wbase);
+    }
+    const int lane = threadIdx.x & (WAVE - 1);
+    const uint32_t prev = __shfl_up(b, 1, WAVE), next = __shfl_down(b, 1, WAVE);
+    const bool head = lane == 0 || prev != b;
+    const bool tail = lane == WAVE - 1 || next != b;
+    const uint64_t heads = __ballot(head), tails = __ballot(tail);
+    const int h = 63 - __clzll((long long)(heads & ((2ull << lane) - 1)));
+    uint32_t base = 0;
+    if (valid && tail) base = atomicAdd(&bcur[b], (uint32_t)(lane - h + 1));
+    // the segment's base, from its tail lane
+    const uint64_t at_or_above = tails & ~((1ull << lane) - 1);
+    const int t = __ffsll((long long)at_or_above) - 1;
+    base = __shfl(base, t, WAVE);
+    if (valid) {
+        const uint32_t o = base + (uint32_t)(lane - h);
+        okeys[o] = key;
+        ovals[o] = v;
+    }
+}
+
+// Per path: the winding entering each row, and the empty-path tile.
+__global__ void k_path_rows_u(const PathGrid* grid, int64_t n_paths, const uint32_t* pwbase, const uint32_t* bbeg,
+                              const int32_t* row_sign, int32_t* row_wb, uint64_t* row_cnt) {
+    const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (p >= n_paths) return;
+    const PathGrid g = grid[p];
+    int32_t running = 0;
+    for (int r = 0; r < g.h; r++) {
+        row_wb[g.row_base + r] = running;
+        running += row_sign[g.row_base + r];
+    }
+    const uint32_t nwin = ((uint32_t)g.w + UW - 1) / UW;
+    const bool nonempty = bbeg[pwbase[p] + (uint32_t)g.h * nwin] > bbeg[pwbase[p]];
+    row_cnt[g.row_base + g.h] = nonempty ? 0ull : (1ull << 32);
+}
+
+struct RowTileArgs {
+    const PathGrid* grid;
+    const uint32_t* row_path;
+    const uint32_t* row_wbase;
+    const int32_t* row_wb;
+    int64_t n_rows;
+    const uint32_t* bbeg;  // bucket starts (n_buckets + 1)
+    const uint32_t* keys;  // bucketed entries
+    const uint32_t* vals;
+    uint32_t xbits;
+    const RunRec* recs;
+    const float4* lines;
+    uint64_t* row_cnt;        // count pass output
+    const uint64_t* row_off;  // emit pass input
+    int16_t* tile_xy;
+    uint8_t* tile_alpha;
+    int16_t* span_xy;
+    uint16_t* span_w;
+    Counters* cnt;
+};
+
+constexpr int RT_NW = 4;
+constexpr int RT_NT = RT_NW * WAVE;
+
+template <bool EMIT>
+struct RowLds {
+    float acc_a[EMIT ? UW * 64 : 1];
+    float acc_h[EMIT ? UW * 64 : 1];
+    int32_t tsign[UW];
+    uint32_t occ[UW / 32];
+    uint8_t tlist[UW];  // the window's occupied tiles in x order
+    uint32_t nt, ns, nocc;
+};
+
+template <bool EMIT>
+__global__ __launch_bounds__(RT_NT) void k_row_tiles(RowTileArgs a) {
+    __shared__ RowLds<EMIT> s;
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1), wid = tid / WAVE;
+    const uint32_t xmask = (1u << a.xbits) - 1u;
+    const uint64_t lt = lanemask_lt();
+    for (int64_t row = blockIdx.x; row < a.n_rows; row += gridDim.x) {
+        const PathGrid G = a.grid[a.row_path[row]];
+        const int32_t r = (int32_t)(row - (int64_t)G.row_base);
+        if (r >= G.h) continue;  // the empty-path row
+        const int32_t ty = G.ty0 + r;
+        const uint32_t nwin = ((uint32_t)G.w + UW - 1) / UW;
+        const uint32_t b0 = a.row_wbase[row];
+        uint32_t row_tiles = 0, row_spans = 0;
+        if (EMIT) {
+            const uint64_t off = a.row_off[row];
+            row_tiles = (uint32_t)(off >> 32);
+            row_spans = (uint32_t)off;
+        }
+        // wave 0's running state along the row
+        int32_t winding = a.row_wb[row];
+        bool have_prev = false;
+        int32_t prev_x = 0, prev_w = 0;
+        uint32_t nt = 0, ns = 0;
+        float carry = 0.0f;  // wave 1, lanes 0-7: the height carry of pixel row `lane`
+        for (uint32_t wi = 0; wi < nwin; wi++) {
+            const uint32_t eb = a.bbeg[b0 + wi], ee = a.bbeg[b0 + wi + 1];
+            if (ee == eb) continue;  // no run and no crossing in this window
+            const int32_t x0 = (int32_t)(wi * UW);
+            if (tid < UW) s.tsign[tid] = 0;
+            if (tid < UW / 32) s.occ[tid] = 0;
+            __syncthreads();
+            // occupancy and crossing sums from the keys alone
+            for (uint32_t e = eb + (uint32_t)tid; e < ee; e += RT_NT) {
+                const uint32_t x = (a.keys[e] & xmask) - (uint32_t)x0;
+                const uint32_t v = a.vals[e];
+                if (v >> 31) {
+                    const int sg = (int)(int8_t)(v & 0xffu);
+                    if (sg) atomicAdd(&s.tsign[x], sg);
+                } else {
+                    atomicOr(&s.occ[x >> 5], 1u << (x & 31));
+                }
+            }
+            __syncthreads();
+            if (wid == 0) {
+                // lane = tile x0 + lane: ranks, the winding after each tile,
+                // and the spans between consecutive tiles (rasterizer.rs:252-264)
+                const bool occ = (s.occ[lane >> 5] >> (lane & 31)) & 1u;
+                const int32_t incl = wave_incl_scan(s.tsign[lane], OpAdd());
+                const int32_t w_at = winding + incl;
+                const uint64_t om = __ballot(occ);
+                const uint64_t below = om & lt;
+                int32_t px = prev_x, pw = prev_w;
+                bool hp = have_prev;
+                const int pl = below ? 63 - __clzll((long long)below) : 0;
+                const int32_t w_pl = __shfl(w_at, pl, WAVE);
+                if (below) {
+                    px = x0 + pl;
+                    pw = w_pl;
+                    hp = true;
+                }
+                const int32_t gx = x0 + lane;
+                const bool span = occ && hp && gx > px + 1 && pw != 0;
+                const uint64_t sm = __ballot(span);
+                if (EMIT && span) {
+                    const uint32_t si = row_spans + ns + (uint32_t)__popcll(sm & lt);
+                    a.span_xy[2 * si] = (int16_t)((G.tx0 + px + 1) * TILE);
+                    a.span_xy[2 * si + 1] = (int16_t)(ty * TILE);
+                    a.span_w[si] = (uint16_t)((uint16_t)(gx - px - 1) * (uint16_t)TILE);
+                }
+                if (EMIT && occ) s.tlist[__popcll(om & lt)] = (uint8_t)lane;
+                if (om) {
+                    const int last = 63 - __clzll((long long)om);
+                    prev_x = x0 + last;
+                    prev_w = __shfl(w_at, last, WAVE);
+                    have_prev = true;
+                }
+                winding = __shfl(w_at, WAVE - 1, WAVE);
+                if (lane == 0) {
+                    s.nt = nt;
+                    s.nocc = (uint32_t)__popcll(om);
+                }
+                nt += (uint32_t)__popcll(om);
+                ns += (uint32_t)__popcll(sm);
+            }
+            if (EMIT) {
+                __syncthreads();
+                const uint32_t nocc = s.nocc, nt_w = s.nt;
+                // zero the occupied tiles' sums, then replay the window's runs into them
+                for (int k = tid; k < UW * 64; k += RT_NT) {
+                    if ((s.occ[k >> 11] >> ((k >> 6) & 31)) & 1u) {
+                        s.acc_a[k] = 0.0f;
+                        s.acc_h[k] = 0.0f;
+                    }
+                }
+                __syncthreads();
+                for (uint32_t e = eb + (uint32_t)tid; e < ee; e += RT_NT) {
+                    const uint32_t v = a.vals[e];
+                    if (v >> 31) continue;
+                    const uint32_t x = (a.keys[e] & xmask) - (uint32_t)x0;
+                    const RunRec rr = a.recs[v];
+                    const float4 L = a.lines[rr.line & 0x7fffffffu];
+                    Dda d;
+                    const V2 last = mk(L.x, L.y), point = mk(L.z, L.w);
+                    if (rr.line >> 31) {
+                        dda_init(d, last, point);
+                    } else {
+                        dda_consts(d, last, point);
+                        d.x = (int16_t)(rr.xy & 0xffffu);
+                        d.y = (int16_t)(rr.xy >> 16);
+                        d.row_t1 = rr.row_t1;
+                        d.col_t1 = rr.col_t1;
+                        d.t0 = 0.0f;
+                        dda_step(d);
+                    }
+                    const int16_t rtx = div8(d.x), rty = div8(d.y);
+                    bool ended = false;
+                    for (int k = 0; k < 16; k++) {
+                        float area, height;
+                        dda_increment(d, area, height);
+                        const uint32_t px = x * 64u + (uint32_t)(((d.y & 7) << 3) | (d.x & 7));
+                        atomicAdd(&s.acc_a[px], area);
+                        atomicAdd(&s.acc_h[px], height);
+                        if (dda_step(d) || div8(d.x) != rtx || div8(d.y) != rty) {
+                            ended = true;
+                            break;
+                        }
+                    }
+                    if (!ended) atomicOr(&a.cnt->error, (uint32_t)ERR_REPLAY);
+                }
+                __syncthreads();
+                if (wid == 1 && lane < 8) {
+                    // the height carry through the window's tiles, left to right
+                    for (int t = 0; t < UW; t++) {
+                        if (!((s.occ[t >> 5] >> (t & 31)) & 1u)) continue;
+                        const int base = t * 64 + lane * 8;
+#pragma unroll
+                        for (int k = 0; k < 8; k++) {
+                            const float h = s.acc_h[base + k];
+                            s.acc_h[base + k] = carry;
+                            carry += h;
+                        }
+                    }
+                }
+                __syncthreads();
+                for (uint32_t item = (uint32_t)tid; item < nocc * 16u; item += RT_NT) {
+                    // item -> (the occupied tile of rank item / 16, 4 pixels)
+                    const uint32_t rk = item >> 4, quad = item & 15u;
+                    const int t = s.tlist[rk];
+                    const int base = t * 64 + (int)quad * 4;
+                    uint32_t word = 0;
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        const float v = s.acc_h[base + k] + s.acc_a[base + k];
+                        word |= (uint32_t)fminf(fabsf(v) * 256.0f, 255.0f) << (8 * k);
+                    }
+                    const uint64_t ti = (uint64_t)row_tiles + nt_w + rk;
+                    reinterpret_cast<uint32_t*>(a.tile_alpha + ti * 64)[quad] = word;
+                    if (quad == 0) {
+                        a.tile_xy[2 * ti] = (int16_t)((G.tx0 + x0 + t) * TILE);
+                        a.tile_xy[2 * ti + 1] = (int16_t)(ty * TILE);
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        if (!EMIT && tid == 0) a.row_cnt[row] = ((uint64_t)nt << 32) | ns;
+    }
+}
+
 // Per-path output ranges and the empty-path tile.
 __global__ void k_path_out(const PathGrid* grid, int64_t n_paths, const uint64_t* row_off, const uint64_t* total,
                            int64_t* tstart, int32_t* tcount, int64_t* sstart, int32_t* scount, int16_t* tile_xy,
@@ -882,7 +1194,16 @@ struct HostTotals {
 // Internal status of one pass: the input is over the per-pass limits.
 constexpr int ST_SPLIT = -100;
 
-static int run_plain_pass(Context* ctx, const SegInput& in, OutputDesc& out, hipStream_t s) {
+struct StoreBucketStart {
+    uint32_t* beg;
+    uint32_t* cur;
+    __device__ void operator()(int64_t i, uint32_t excl, uint32_t) const {
+        beg[i] = excl;
+        cur[i] = excl;
+    }
+};
+
+static int run_plain_pass(Context* ctx, const SegInput& in, OutputDesc& out, hipStream_t s, bool unordered) {
     const int64_t V = in.n_verbs;
     const int64_t NP = in.n_paths;
     if (NP <= 0) {
@@ -1018,6 +1339,103 @@ static int run_plain_pass(Context* ctx, const SegInput& in, OutputDesc& out, hip
     da.vals = ctx->vals[0].as<uint32_t>();
     da.recs = ctx->records.as<RunRec>();
     if (L > 0) hipLaunchKernelGGL(k_dda<true>, dim3(grid_for(L, 256)), dim3(256), 0, s, da);
+    if (unordered) {
+        // ---- window buckets instead of the sort
+        uint32_t* keys0 = ctx->keys[0].as<uint32_t>();
+        uint32_t* vals0 = ctx->vals[0].as<uint32_t>();
+        HIPCHK(ctx->u_pwbase.ensure((size_t)NP * 4));
+        HIPCHK(ctx->u_rwbase.ensure((size_t)ROWS * 4));
+        HIPCHK(ctx->row_path.ensure(ROWS * 4));
+        HIPCHK(ctx->row_sign.ensure(ROWS * 4));
+        HIPCHK(ctx->row_wb.ensure(ROWS * 4));
+        HIPCHK(ctx->row_cnt.ensure(ROWS * 8));
+        HIPCHK(ctx->row_off.ensure(ROWS * 8));
+        uint32_t* pwbase = ctx->u_pwbase.as<uint32_t>();
+        uint32_t* rwbase = ctx->u_rwbase.as<uint32_t>();
+        uint32_t* nb_dev = reinterpret_cast<uint32_t*>(&tot[6]);
+        HIPCHK(launch_scan<uint32_t>(ctx, NP, LoadNWin{grid}, StoreU32{pwbase}, OpAdd(), 0u, nb_dev, s));
+        hipLaunchKernelGGL(k_rows_setup, dim3(grid_for(NP, 256)), dim3(256), 0, s, grid, NP, pwbase,
+                           ctx->row_path.as<uint32_t>(), rwbase);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(&ht->groups, nb_dev, 4, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        const int64_t NB = ht->groups;
+        HIPCHK(ctx->u_bcnt.ensure((size_t)std::max<int64_t>(NB, 1) * 4));
+        HIPCHK(ctx->u_bbeg.ensure((size_t)(NB + 1) * 4));
+        HIPCHK(ctx->u_bcur.ensure((size_t)std::max<int64_t>(NB, 1) * 4));
+        HIPCHK(ensure_scan_capacity(ctx, std::max<int64_t>({NB, ROWS, 1})));
+        uint32_t* bcnt = ctx->u_bcnt.as<uint32_t>();
+        uint32_t* bbeg = ctx->u_bbeg.as<uint32_t>();
+        uint32_t* bcur = ctx->u_bcur.as<uint32_t>();
+        fill<uint32_t>(bcnt, std::max<int64_t>(NB, 1), 0u, s);
+        fill<int32_t>(ctx->row_sign.as<int32_t>(), ROWS, 0, s);
+        if (R > 0)
+            hipLaunchKernelGGL(k_bucket_hist, dim3(grid_for(R, 256)), dim3(256), 0, s, keys0, vals0, R, xbits, rwbase,
+                               bcnt, ctx->row_sign.as<int32_t>());
+        if (NB > 0)
+            HIPCHK(launch_scan<uint32_t>(ctx, NB, LoadU32{bcnt}, StoreBucketStart{bbeg, bcur}, OpAdd(), 0u, bbeg + NB,
+                                         s));
+        else
+            fill<uint32_t>(bbeg, 1, 0u, s);
+        if (R > 0)
+            hipLaunchKernelGGL(k_bucket_scatter, dim3(grid_for(R, 256)), dim3(256), 0, s, keys0, vals0, R, xbits,
+                               rwbase, bcur, ctx->keys[1].as<uint32_t>(), ctx->vals[1].as<uint32_t>());
+        hipLaunchKernelGGL(k_path_rows_u, dim3(grid_for(NP, 256)), dim3(256), 0, s, grid, NP, pwbase, bbeg,
+                           ctx->row_sign.as<int32_t>(), ctx->row_wb.as<int32_t>(), ctx->row_cnt.as<uint64_t>());
+        RowTileArgs ra;
+        ra.grid = grid;
+        ra.row_path = ctx->row_path.as<uint32_t>();
+        ra.row_wbase = rwbase;
+        ra.row_wb = ctx->row_wb.as<int32_t>();
+        ra.n_rows = ROWS;
+        ra.bbeg = bbeg;
+        ra.keys = ctx->keys[1].as<uint32_t>();
+        ra.vals = ctx->vals[1].as<uint32_t>();
+        ra.xbits = xbits;
+        ra.recs = ctx->records.as<RunRec>();
+        ra.lines = ctx->lines.as<float4>();
+        ra.row_cnt = ctx->row_cnt.as<uint64_t>();
+        ra.row_off = ctx->row_off.as<uint64_t>();
+        ra.tile_xy = out.tile_xy;
+        ra.tile_alpha = out.tile_alpha;
+        ra.span_xy = out.span_xy;
+        ra.span_w = out.span_w;
+        ra.cnt = cnt;
+        const uint32_t row_blocks = (uint32_t)std::min<int64_t>(ROWS, 1 << 16);
+        hipLaunchKernelGGL(k_row_tiles<false>, dim3(row_blocks), dim3(RT_NT), 0, s, ra);
+        HIPCHK(launch_scan<uint64_t>(ctx, ROWS, LoadU64{ra.row_cnt}, StoreU64{ctx->row_off.as<uint64_t>()}, OpAdd(),
+                                     0ull, &tot[4], s));
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(&ht->out, &tot[4], 8, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(&ht->error, &cnt->error, 4, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        if (ht->error & (ERR_BBOX | ERR_REPLAY)) return -3;
+        const int64_t T = (int64_t)(ht->out >> 32);
+        const int64_t S = (int64_t)(uint32_t)ht->out;
+        out.n_tiles = T;
+        out.n_spans = S;
+        if (T > out.tile_capacity || S > out.span_capacity) return -4;
+        if (ctx->timing) {
+            if (!ctx->ev[0]) HIPCHK(hipEventCreate(&ctx->ev[0]));
+            if (!ctx->ev[1]) HIPCHK(hipEventCreate(&ctx->ev[1]));
+            HIPCHK(hipEventRecord(ctx->ev[0], s));
+        }
+        hipLaunchKernelGGL(k_row_tiles<true>, dim3(row_blocks), dim3(RT_NT), 0, s, ra);
+        if (ctx->timing) HIPCHK(hipEventRecord(ctx->ev[1], s));
+        hipLaunchKernelGGL(k_path_out, dim3(grid_for(NP, 256)), dim3(256), 0, s, grid, NP, ra.row_off,
+                           (const uint64_t*)&tot[4], out.path_tile_start, out.path_tile_count, out.path_span_start,
+                           out.path_span_count, out.tile_xy, out.tile_alpha, out.tile_base, out.span_base);
+        HIPCHK(hipGetLastError());
+        if (ctx->timing) {
+            HIPCHK(hipEventSynchronize(ctx->ev[1]));
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+            auto& t = ctx->timings["k_row_tiles"];
+            t.first += ms;
+            t.second += 1;
+        }
+        return 0;
+    }
     int cur = 0;
     const int passes = (int)((kbits + 7) / 8);
     uint32_t* rsc = ctx->rs_counts.as<uint32_t>();
@@ -1144,8 +1562,8 @@ __global__ void k_rebase_offsets(const int64_t* voff, const int64_t* poff, const
     dst[2 * (n + 1) + i] = woff[p0 + i] - woff[p0];
 }
 
-int run_plain(Context* ctx, const SegInput& in, OutputDesc& out, hipStream_t s) {
-    int st = run_plain_pass(ctx, in, out, s);
+int run_plain(Context* ctx, const SegInput& in, OutputDesc& out, hipStream_t s, bool unordered) {
+    int st = run_plain_pass(ctx, in, out, s, unordered);
     if (st != ST_SPLIT) return st;
     // Paths are independent Rasterizers: rasterise path ranges one after
     // another, each range's tiles and spans following the previous range's.
@@ -1213,7 +1631,7 @@ int run_plain(Context* ctx, const SegInput& in, OutputDesc& out, hipStream_t s) 
         od.path_span_count = out.path_span_count + p0;
         od.tile_base = out.tile_base + T;
         od.span_base = out.span_base + S;
-        st = run_plain_pass(ctx, si, od, s);
+        st = run_plain_pass(ctx, si, od, s, unordered);
         if (st == ST_SPLIT) {
             if (n < 2) return -7;
             cut(p0, p1);

@@ -228,18 +228,28 @@ def build_atlas(ctx: Context, out: DeviceOutput, colors, atlas_size=4096, stream
 def run_to_host(ctx, batch, mode="plain", tile_capacity=None, span_capacity=None):
     """Convenience: upload, rasterise (growing the outputs once if needed)
     and return the per-path host arrays.  Modes: "plain", "batched" (the
-    ordered batched kernel alone), "dense" (the dense kernel in front of it),
-    "stroke_plain", "stroke_batched" (strokes use batch.widths)."""
+    ordered batched kernel alone), "dense" (the dense kernels in front of it),
+    and the same for strokes (batch.widths): "stroke_plain",
+    "stroke_batched", "stroke_dense".  The context's dense setting is
+    restored afterwards."""
+    d = dense_params(ctx)
+    if mode in ("batched", "dense", "stroke_batched", "stroke_dense"):
+        set_dense_params(ctx, mode in ("dense", "stroke_dense"), d["dense_groups_per_cu"],
+                         d["dense_waves_per_group"])
+    try:
+        return _run_to_host(ctx, batch, mode, tile_capacity, span_capacity)
+    finally:
+        set_dense_params(ctx, bool(d["dense"]), d["dense_groups_per_cu"], d["dense_waves_per_group"])
+
+
+def _run_to_host(ctx, batch, mode, tile_capacity, span_capacity):
     torch = _torch()
-    if mode in ("batched", "dense"):
-        d = dense_params(ctx)
-        set_dense_params(ctx, mode == "dense", d["dense_groups_per_cu"], d["dense_waves_per_group"])
     db = upload(batch)
     tc = tile_capacity or max(1024, batch.n_paths * 64)
     sc = span_capacity or max(1024, batch.n_paths * 8)
     if mode.startswith("stroke"):
         wd = torch.from_numpy(np.ascontiguousarray(batch.widths, np.float32)).to("cuda")
-        fn = lambda c, d, o: stroke(c, d, wd, o, batched=mode == "stroke_batched")  # noqa: E731
+        fn = lambda c, d, o: stroke(c, d, wd, o, batched=mode != "stroke_plain")  # noqa: E731
     else:
         fn = rasterize_plain if mode == "plain" else rasterize_batched
     for _ in range(2):

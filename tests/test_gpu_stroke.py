@@ -1,6 +1,8 @@
 """Parity of device stroking (flatten + stroke expansion on the GPU, then the
 plain or batched fill) with the oracle's Rasterizer::stroke, through the C
-ABI: identical tile and span sets, alpha bit-exact."""
+ABI: identical tile and span sets; alpha bit-exact for the plain pipeline
+and the ordered batched kernel, within 1 LSB for the dense kernels
+(arrival-order f32 sums), whose bit-exact fraction must stay >= 0.999."""
 import numpy as np
 import pytest
 
@@ -10,7 +12,7 @@ from parity import compare
 
 pytestmark = pytest.mark.gpu
 
-MODES = ["stroke_plain", "stroke_batched"]
+MODES = ["stroke_plain", "stroke_batched", "stroke_dense"]
 
 
 def check(ctx, batch, mode):
@@ -19,8 +21,10 @@ def check(ctx, batch, mode):
     ref = oracle_ref.run_batch(batch, stroke=True)
     assert ref.status == 0
     got = run_to_host(ctx, batch, mode)
-    st = compare(ref, got, max_lsb=0)
+    st = compare(ref, got, max_lsb=1 if mode == "stroke_dense" else 0)
     print(f"[{mode}] {batch.name}: {st}", flush=True)
+    if mode == "stroke_dense":
+        assert st["bit_exact_fraction"] >= 0.999
     return st
 
 
