@@ -871,62 +871,98 @@ __device__ inline uint32_t entry_bucket(uint32_t key, uint32_t xbits, const uint
     return row_wbase[key >> xbits] + ((key & ((1u << xbits) - 1u)) / UW);
 }
 
-// Bucket sizes (one atomic per run of equal buckets in a wave) and the
-// crossing sum of every tile row.
-__global__ __launch_bounds__(256) void k_bucket_hist(const uint32_t* keys, const uint32_t* vals, int64_t R,
-                                                     uint32_t xbits, const uint32_t* row_wbase, uint32_t* bcnt,
-                                                     int32_t* row_sign) {
-    const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    const bool valid = e < R;
-    uint32_t b = 0xffffffffu;
-    if (valid) {
-        const uint32_t key = keys[e], v = vals[e];
-        b = entry_bucket(key, xbits, row_wbase);
-        if (v >> 31) {
-            const int sg = (int)(int8_t)(v & 0xffu);
-            if (sg) atomicAdd(&row_sign[key >> xbits], sg);
+// Bucketing: each workgroup takes BK_ITEMS entries per thread and first
+// counts them per bucket in an LDS hash table (open addressing, room for every
+// entry), so that a bucket gets one global atomic per workgroup rather than
+// one per entry: the entries of a workgroup are consecutive in emission order
+// (the runs and crossings of a few neighbouring lines) and fall into few
+// buckets.
+constexpr int BK_NT = 256;
+constexpr int BK_ITEMS = 2;
+constexpr int BK_HS = 1024;  // >= BK_NT * BK_ITEMS
+constexpr uint32_t BK_EMPTY = 0xffffffffu;
+
+struct BucketLds {
+    uint32_t key[BK_HS];
+    uint32_t cnt[BK_HS];
+    uint32_t base[BK_HS];
+};
+
+// Inserts bucket b; returns its slot and the entry's rank within the block.
+__device__ inline uint32_t bucket_insert(BucketLds& h, uint32_t b, uint32_t* rank) {
+    uint32_t slot = (b * 2654435761u) >> (32 - 10);  // BK_HS = 2^10
+    for (;;) {
+        const uint32_t old = atomicCAS(&h.key[slot], BK_EMPTY, b);
+        if (old == BK_EMPTY || old == b) {
+            *rank = atomicAdd(&h.cnt[slot], 1u);
+            return slot;
         }
-    }
-    const int lane = threadIdx.x & (WAVE - 1);
-    const uint32_t prev = __shfl_up(b, 1, WAVE), next = __shfl_down(b, 1, WAVE);
-    const bool head = lane == 0 || prev != b;
-    const bool tail = lane == WAVE - 1 || next != b;
-    const uint64_t heads = __ballot(head);
-    if (valid && tail) {
-        const uint64_t below = heads & ((2ull << lane) - 1);  // my segment's head: highest head <= lane
-        const int h = 63 - __clzll((long long)below);
-        atomicAdd(&bcnt[b], (uint32_t)(lane - h + 1));
+        slot = (slot + 1) & (BK_HS - 1);
     }
 }
 
-// Scatter into the buckets (one cursor atomic per run of equal buckets).
-__global__ __launch_bounds__(256) void k_bucket_scatter(const uint32_t* keys, const uint32_t* vals, int64_t R,
-                                                        uint32_t xbits, const uint32_t* row_wbase, uint32_t* bcur,
-                                                        uint32_t* okeys, uint32_t* ovals) {
-    const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    const bool valid = e < R;
-    uint32_t b = 0xffffffffu, key = 0, v = 0;
-    if (valid) {
-        key = keys[e];
-        v = vals[e];
-        b = entry_bucket(key, xbits, row_wbase);
+// Bucket sizes, and the crossing sum of every tile row.
+__global__ __launch_bounds__(BK_NT) void k_bucket_hist(const uint32_t* keys, const uint32_t* vals, int64_t R,
+                                                       uint32_t xbits, const uint32_t* row_wbase, uint32_t* bcnt,
+                                                       int32_t* row_sign) {
+    __shared__ BucketLds h;
+    for (int k = threadIdx.x; k < BK_HS; k += BK_NT) {
+        h.key[k] = BK_EMPTY;
+        h.cnt[k] = 0;
     }
-    const int lane = threadIdx.x & (WAVE - 1);
-    const uint32_t prev = __shfl_up(b, 1, WAVE), next = __shfl_down(b, 1, WAVE);
-    const bool head = lane == 0 || prev != b;
-    const bool tail = lane == WAVE - 1 || next != b;
-    const uint64_t heads = __ballot(head), tails = __ballot(tail);
-    const int h = 63 - __clzll((long long)(heads & ((2ull << lane) - 1)));
-    uint32_t base = 0;
-    if (valid && tail) base = atomicAdd(&bcur[b], (uint32_t)(lane - h + 1));
-    // the segment's base, from its tail lane
-    const uint64_t at_or_above = tails & ~((1ull << lane) - 1);
-    const int t = __ffsll((long long)at_or_above) - 1;
-    base = __shfl(base, t, WAVE);
-    if (valid) {
-        const uint32_t o = base + (uint32_t)(lane - h);
-        okeys[o] = key;
-        ovals[o] = v;
+    __syncthreads();
+    const int64_t e0 = (int64_t)blockIdx.x * BK_NT * BK_ITEMS + threadIdx.x;
+#pragma unroll
+    for (int u = 0; u < BK_ITEMS; u++) {
+        const int64_t e = e0 + (int64_t)u * BK_NT;
+        if (e < R) {
+            const uint32_t key = keys[e], v = vals[e];
+            uint32_t rank;
+            (void)bucket_insert(h, entry_bucket(key, xbits, row_wbase), &rank);
+            if (v >> 31) {
+                const int sg = (int)(int8_t)(v & 0xffu);
+                if (sg) atomicAdd(&row_sign[key >> xbits], sg);
+            }
+        }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < BK_HS; k += BK_NT)
+        if (h.key[k] != BK_EMPTY) atomicAdd(&bcnt[h.key[k]], h.cnt[k]);
+}
+
+// Scatter into the buckets: one cursor atomic per bucket per workgroup.
+__global__ __launch_bounds__(BK_NT) void k_bucket_scatter(const uint32_t* keys, const uint32_t* vals, int64_t R,
+                                                          uint32_t xbits, const uint32_t* row_wbase, uint32_t* bcur,
+                                                          uint32_t* okeys, uint32_t* ovals) {
+    __shared__ BucketLds h;
+    for (int k = threadIdx.x; k < BK_HS; k += BK_NT) {
+        h.key[k] = BK_EMPTY;
+        h.cnt[k] = 0;
+    }
+    __syncthreads();
+    const int64_t e0 = (int64_t)blockIdx.x * BK_NT * BK_ITEMS + threadIdx.x;
+    uint32_t kk[BK_ITEMS], vv[BK_ITEMS], slot[BK_ITEMS], rank[BK_ITEMS];
+#pragma unroll
+    for (int u = 0; u < BK_ITEMS; u++) {
+        const int64_t e = e0 + (int64_t)u * BK_NT;
+        slot[u] = BK_EMPTY;
+        if (e < R) {
+            kk[u] = keys[e];
+            vv[u] = vals[e];
+            slot[u] = bucket_insert(h, entry_bucket(kk[u], xbits, row_wbase), &rank[u]);
+        }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < BK_HS; k += BK_NT)
+        if (h.key[k] != BK_EMPTY) h.base[k] = atomicAdd(&bcur[h.key[k]], h.cnt[k]);
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < BK_ITEMS; u++) {
+        if (slot[u] != BK_EMPTY) {
+            const uint32_t o = h.base[slot[u]] + rank[u];
+            okeys[o] = kk[u];
+            ovals[o] = vv[u];
+        }
     }
 }
 
@@ -969,6 +1005,7 @@ struct RowTileArgs {
 
 constexpr int RT_NW = 4;
 constexpr int RT_NT = RT_NW * WAVE;
+constexpr int RT_U = 4;  // entries per thread whose loads are issued together
 
 template <bool EMIT>
 struct RowLds {
@@ -1012,15 +1049,26 @@ __global__ __launch_bounds__(RT_NT) void k_row_tiles(RowTileArgs a) {
             if (tid < UW) s.tsign[tid] = 0;
             if (tid < UW / 32) s.occ[tid] = 0;
             __syncthreads();
-            // occupancy and crossing sums from the keys alone
-            for (uint32_t e = eb + (uint32_t)tid; e < ee; e += RT_NT) {
-                const uint32_t x = (a.keys[e] & xmask) - (uint32_t)x0;
-                const uint32_t v = a.vals[e];
-                if (v >> 31) {
-                    const int sg = (int)(int8_t)(v & 0xffu);
-                    if (sg) atomicAdd(&s.tsign[x], sg);
-                } else {
-                    atomicOr(&s.occ[x >> 5], 1u << (x & 31));
+            // occupancy and crossing sums from the keys alone, RT_U entries
+            // per thread at a time (their loads in flight together)
+            for (uint32_t e0 = eb + (uint32_t)tid; e0 < ee; e0 += RT_U * RT_NT) {
+                uint32_t kk[RT_U], vv[RT_U];
+#pragma unroll
+                for (int u = 0; u < RT_U; u++) {
+                    const uint32_t e = e0 + (uint32_t)(u * RT_NT);
+                    kk[u] = e < ee ? a.keys[e] : 0u;
+                    vv[u] = e < ee ? a.vals[e] : 0u;
+                }
+#pragma unroll
+                for (int u = 0; u < RT_U; u++) {
+                    if (e0 + (uint32_t)(u * RT_NT) >= ee) break;
+                    const uint32_t x = (kk[u] & xmask) - (uint32_t)x0;
+                    if (vv[u] >> 31) {
+                        const int sg = (int)(int8_t)(vv[u] & 0xffu);
+                        if (sg) atomicAdd(&s.tsign[x], sg);
+                    } else {
+                        atomicOr(&s.occ[x >> 5], 1u << (x & 31));
+                    }
                 }
             }
             __syncthreads();
@@ -1076,12 +1124,34 @@ __global__ __launch_bounds__(RT_NT) void k_row_tiles(RowTileArgs a) {
                     }
                 }
                 __syncthreads();
-                for (uint32_t e = eb + (uint32_t)tid; e < ee; e += RT_NT) {
-                    const uint32_t v = a.vals[e];
-                    if (v >> 31) continue;
-                    const uint32_t x = (a.keys[e] & xmask) - (uint32_t)x0;
-                    const RunRec rr = a.recs[v];
-                    const float4 L = a.lines[rr.line & 0x7fffffffu];
+                for (uint32_t e0 = eb + (uint32_t)tid; e0 < ee; e0 += RT_U * RT_NT) {
+                  // the RT_U entries' values, run records and lines are loaded
+                  // level by level, each level's loads in flight together
+                  uint32_t vv[RT_U], xx[RT_U];
+                  RunRec rrs[RT_U];
+                  float4 Ls[RT_U];
+#pragma unroll
+                  for (int u = 0; u < RT_U; u++) {
+                      const uint32_t e = e0 + (uint32_t)(u * RT_NT);
+                      vv[u] = e < ee ? a.vals[e] : 0x80000000u;
+                      xx[u] = e < ee ? (a.keys[e] & xmask) - (uint32_t)x0 : 0u;
+                  }
+#pragma unroll
+                  for (int u = 0; u < RT_U; u++) {
+                      rrs[u].line = 0;
+                      rrs[u].xy = 0;
+                      rrs[u].row_t1 = rrs[u].col_t1 = 0.0f;
+                      if (!(vv[u] >> 31)) rrs[u] = a.recs[vv[u]];
+                  }
+#pragma unroll
+                  for (int u = 0; u < RT_U; u++)
+                      Ls[u] = !(vv[u] >> 31) ? a.lines[rrs[u].line & 0x7fffffffu] : make_float4(0, 0, 0, 0);
+#pragma unroll
+                  for (int u = 0; u < RT_U; u++) {
+                    if (vv[u] >> 31) continue;
+                    const uint32_t x = xx[u];
+                    const RunRec rr = rrs[u];
+                    const float4 L = Ls[u];
                     Dda d;
                     const V2 last = mk(L.x, L.y), point = mk(L.z, L.w);
                     if (rr.line >> 31) {
@@ -1109,6 +1179,7 @@ __global__ __launch_bounds__(RT_NT) void k_row_tiles(RowTileArgs a) {
                         }
                     }
                     if (!ended) atomicOr(&a.cnt->error, (uint32_t)ERR_REPLAY);
+                  }
                 }
                 __syncthreads();
                 if (wid == 1 && lane < 8) {
@@ -1369,8 +1440,9 @@ static int run_plain_pass(Context* ctx, const SegInput& in, OutputDesc& out, hip
         uint32_t* bcur = ctx->u_bcur.as<uint32_t>();
         fill<uint32_t>(bcnt, std::max<int64_t>(NB, 1), 0u, s);
         fill<int32_t>(ctx->row_sign.as<int32_t>(), ROWS, 0, s);
+        const uint32_t bk_blocks = grid_for(R, BK_NT * BK_ITEMS);
         if (R > 0)
-            hipLaunchKernelGGL(k_bucket_hist, dim3(grid_for(R, 256)), dim3(256), 0, s, keys0, vals0, R, xbits, rwbase,
+            hipLaunchKernelGGL(k_bucket_hist, dim3(bk_blocks), dim3(BK_NT), 0, s, keys0, vals0, R, xbits, rwbase,
                                bcnt, ctx->row_sign.as<int32_t>());
         if (NB > 0)
             HIPCHK(launch_scan<uint32_t>(ctx, NB, LoadU32{bcnt}, StoreBucketStart{bbeg, bcur}, OpAdd(), 0u, bbeg + NB,
@@ -1378,8 +1450,8 @@ static int run_plain_pass(Context* ctx, const SegInput& in, OutputDesc& out, hip
         else
             fill<uint32_t>(bbeg, 1, 0u, s);
         if (R > 0)
-            hipLaunchKernelGGL(k_bucket_scatter, dim3(grid_for(R, 256)), dim3(256), 0, s, keys0, vals0, R, xbits,
-                               rwbase, bcur, ctx->keys[1].as<uint32_t>(), ctx->vals[1].as<uint32_t>());
+            hipLaunchKernelGGL(k_bucket_scatter, dim3(bk_blocks), dim3(BK_NT), 0, s, keys0, vals0, R, xbits, rwbase,
+                               bcur, ctx->keys[1].as<uint32_t>(), ctx->vals[1].as<uint32_t>());
         hipLaunchKernelGGL(k_path_rows_u, dim3(grid_for(NP, 256)), dim3(256), 0, s, grid, NP, pwbase, bbeg,
                            ctx->row_sign.as<int32_t>(), ctx->row_wb.as<int32_t>(), ctx->row_cnt.as<uint64_t>());
         RowTileArgs ra;
