@@ -1014,6 +1014,7 @@ struct RowLds {
     int32_t tsign[UW];
     uint32_t occ[UW / 32];
     uint8_t tlist[UW];  // the window's occupied tiles in x order
+    float rtot[EMIT ? UW * 8 : 1];  // per (occupied tile, pixel row): height total, then incoming carry
     uint32_t nt, ns, nocc;
 };
 
@@ -1182,17 +1183,28 @@ __global__ __launch_bounds__(RT_NT) void k_row_tiles(RowTileArgs a) {
                   }
                 }
                 __syncthreads();
-                if (wid == 1 && lane < 8) {
-                    // the height carry through the window's tiles, left to right
-                    for (int t = 0; t < UW; t++) {
-                        if (!((s.occ[t >> 5] >> (t & 31)) & 1u)) continue;
-                        const int base = t * 64 + lane * 8;
+                // The height carry: every (occupied tile, pixel row) folds its 8
+                // heights into an in-tile prefix and a total, 8 lanes then carry
+                // the totals through the window's tiles left to right, and the
+                // quantise adds the two.  (Regrouping the row's f32 sums this way
+                // moves a pixel's carry by a few ulps: within the 1-LSB contract.)
+                for (uint32_t item = (uint32_t)tid; item < nocc * 8u; item += RT_NT) {
+                    const int base = (int)s.tlist[item >> 3] * 64 + (int)(item & 7u) * 8;
+                    float acc = 0.0f;
 #pragma unroll
-                        for (int k = 0; k < 8; k++) {
-                            const float h = s.acc_h[base + k];
-                            s.acc_h[base + k] = carry;
-                            carry += h;
-                        }
+                    for (int k = 0; k < 8; k++) {
+                        const float h = s.acc_h[base + k];
+                        s.acc_h[base + k] = acc;
+                        acc += h;
+                    }
+                    s.rtot[item] = acc;
+                }
+                __syncthreads();
+                if (wid == 1 && lane < 8) {
+                    for (uint32_t rk = 0; rk < nocc; rk++) {
+                        const float t = s.rtot[rk * 8 + lane];
+                        s.rtot[rk * 8 + lane] = carry;
+                        carry += t;
                     }
                 }
                 __syncthreads();
@@ -1202,9 +1214,10 @@ __global__ __launch_bounds__(RT_NT) void k_row_tiles(RowTileArgs a) {
                     const int t = s.tlist[rk];
                     const int base = t * 64 + (int)quad * 4;
                     uint32_t word = 0;
+                    const float cin = s.rtot[rk * 8 + (quad >> 1)];  // pixel row quad / 2
 #pragma unroll
                     for (int k = 0; k < 4; k++) {
-                        const float v = s.acc_h[base + k] + s.acc_a[base + k];
+                        const float v = (cin + s.acc_h[base + k]) + s.acc_a[base + k];
                         word |= (uint32_t)fminf(fabsf(v) * 256.0f, 255.0f) << (8 * k);
                     }
                     const uint64_t ti = (uint64_t)row_tiles + nt_w + rk;
