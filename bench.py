@@ -188,6 +188,81 @@ def traffic_bytes(kernel, workload):
     return None
 
 
+def adjoint_bench(args):
+    """The differentiable configuration: loss and gradients (w.r.t. every
+    control point and transform) of 4,096 16-cubic glyphs against a target
+    rendered from control points perturbed by N(0, 0.5 px), through
+    ochre_coverage_adjoint (the forward runs inside it).  The target render is
+    made once, before the timed steps.  CPU baseline: the f64 oracle adjoint
+    of the same formula on one core, on a sample of paths."""
+    import numpy as np
+    import torch
+    from ochre_amd import workloads as W
+    from ochre_amd.device import Context, DeviceOutput, coverage_adjoint, rasterize_batched, upload
+    batch = W.adjoint_glyphs()
+    rng = np.random.default_rng(900)
+    tb = W.PathBatch(batch.verbs, batch.verb_off, batch.points.copy(), batch.point_off, batch.weights,
+                     batch.weight_off, batch.xforms, name=batch.name + "[target]")
+    scale = np.repeat(batch.xforms[:, 0], np.diff(batch.point_off))[:, None]
+    tb.points += (rng.normal(0, 0.5, tb.points.shape) / scale).astype(np.float32)
+    ctx = Context(0)
+    db = upload(batch)
+    tgt = DeviceOutput(tb.n_paths, tb.n_paths * 128, tb.n_paths * 16, coverage=True)
+    rasterize_batched(ctx, upload(tb), tgt)
+    torch.cuda.synchronize()
+    fn = lambda: coverage_adjoint(ctx, db, tgt)  # noqa: E731
+    ctx.set_timing(True)
+    ms, wall = time_steps(fn, args.steps, args.warmup)
+    kt, kn = ctx.timing("k_adjoint")
+    ctx.set_timing(False)
+    kms = kt / max(kn, 1)
+    loss, dp, dx = fn()
+    torch.cuda.synchronize()
+    # interface bytes: the input paths, the target's f32 coverage, the
+    # gradients (8 B per point, 24 B per transform) and the loss per path
+    b_in = batch.algorithmic_input_bytes() + 4 * 64 * tgt.n_tiles + 6 * tgt.n_spans
+    b_out = 8 * batch.points.shape[0] + 24 * batch.n_paths + 4 * batch.n_paths
+    achieved = (b_in + b_out) / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
+    log(f"adjoint: {ms:.3f} ms/step, {batch.n_paths / ms * 1e3:,.0f} paths/s, k_adjoint {kms:.3f} ms, "
+        f"loss {float(loss.sum()):.4f}")
+    line = {"metric": "adjoint paths/s (f32 coverage loss and gradients w.r.t. control points and transforms)",
+            "value": batch.n_paths / ms * 1e3, "unit": "paths/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": dict({"workload": batch.name, "paths": batch.n_paths, "target": "control points + N(0, 0.5 px)",
+                            "target_tiles": tgt.n_tiles}, **WORKLOAD_INFO["adjoint_glyphs"]),
+            "kernels_ms": {"k_adjoint": kms},
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_adjoint", "kernel_ms": kms,
+                         "algorithmic_bytes": b_in + b_out}}
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_ref as O
+        try:
+            os.sched_setaffinity(0, {sorted(os.sched_getaffinity(0))[0]})
+        except Exception:
+            pass
+        txy, tcov = tgt.tile_xy.cpu().numpy(), tgt.tile_cov.cpu().numpy()
+        sxy, sw = tgt.span_xy.cpu().numpy(), tgt.span_w.cpu().numpy().view(np.uint16)
+        ts, tc = tgt.path_tile_start.cpu().numpy(), tgt.path_tile_count.cpu().numpy()
+        ss, sc = tgt.path_span_start.cpu().numpy(), tgt.path_span_count.cpu().numpy()
+        t0 = time.perf_counter()
+        done = 0
+        while done < batch.n_paths and (time.perf_counter() - t0 < args.cpu_seconds or done == 0):
+            v, p, w, xf = batch.path(done)
+            t = O.Target(txy[ts[done]:ts[done] + tc[done]], tcov[ts[done]:ts[done] + tc[done]],
+                         sxy[ss[done]:ss[done] + sc[done]], sw[ss[done]:ss[done] + sc[done]])
+            O.path_loss64(v, p, w, xf, t, grad=True)
+            done += 1
+        el = time.perf_counter() - t0
+        line["cpu_baseline"] = {"value": done / el, "unit": "paths/s", "cores": 1, "kind": "port",
+                                "sample": f"{batch.name}: the first {done} paths, f64 oracle adjoint (single thread)"}
+        line["vs_cpu_1core"] = line["value"] / line["cpu_baseline"]["value"]
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+
+
 def time_steps(fn, steps, warmup):
     import torch
     for _ in range(warmup):
@@ -222,6 +297,8 @@ def main():
                     help="workgroups per CU of the small-grid dense instance (0: off) instead of autotuning")
     ap.add_argument("--no-dense", action="store_true", help="the ordered batched kernel alone (bit-exact sums)")
     ap.add_argument("--no-graph", action="store_true", help="launch the batched kernel directly, not from a graph")
+    ap.add_argument("--adjoint", action="store_true",
+                    help="time the coverage adjoint on the differentiable configuration instead")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 5 times (scales rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -243,6 +320,10 @@ def main():
         print(f"bench.py measures one GPU: this build's path runs on a single MI355X "
               f"(--gpus {args.gpus} is not supported)", file=sys.stderr)
         sys.exit(2)
+
+    if args.adjoint:
+        adjoint_bench(args)
+        return
 
     import torch
 

@@ -90,6 +90,13 @@ def load() -> ctypes.CDLL:
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"{LIB_PATH} is missing; build it with `python -m ochre_amd.build`")
+    # torch's HIP runtime is the one that owns the device memory handed to the
+    # library: load it first, so that the library binds to that runtime rather
+    # than to a second copy from the system ROCm
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(LIB_PATH)
     P, i64, f32 = ctypes.c_void_p, ctypes.c_int64, ctypes.c_float
     lib.ochre_rasterizer_new.restype = P

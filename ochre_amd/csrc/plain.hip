@@ -691,12 +691,16 @@ __global__ __launch_bounds__(WAVE) void k_tile_counts(TileArgs a) {
 // left and the winding after it is non-zero (rasterizer.rs:252-264).
 constexpr int TR_SLOTS = 16;  // increments per run slot
 constexpr int TR_BLOCK = 32;  // runs replayed per block (6 KB of LDS per wave)
+// (Padding these rows against the lane-strided bank conflicts of the replay
+// was measured slower: the replay is a small part of the kernel.)
+constexpr int TR_SSTR = TR_SLOTS;  // float2 per run slot row
+constexpr int TR_MSTR = 64;        // bytes per run map row
 
 __device__ inline uint32_t rl(uint32_t v, int j) { return (uint32_t)__builtin_amdgcn_readlane((int)v, j); }
 
 __global__ __launch_bounds__(WAVE) void k_tile_rows(TileArgs a) {
-    __shared__ float2 inc[TR_BLOCK * TR_SLOTS];
-    __shared__ __align__(16) uint8_t smap[TR_BLOCK * 64];  // run j, pixel p -> step + 1 (0: not visited)
+    __shared__ float2 inc[TR_BLOCK * TR_SSTR];
+    __shared__ __align__(16) uint8_t smap[TR_BLOCK * TR_MSTR];  // run j, pixel p -> step + 1 (0: not visited)
     const int lane = threadIdx.x;
     const uint32_t xmask = (1u << a.xbits) - 1;
     for (int64_t row = blockIdx.x; row < a.n_rows; row += gridDim.x) {
@@ -757,9 +761,9 @@ __global__ __launch_bounds__(WAVE) void k_tile_rows(TileArgs a) {
                     ebase = e;
                     __syncthreads();  // the previous block's maps are read
                     if (lane < TR_BLOCK) {
-                        uint4* row_map = reinterpret_cast<uint4*>(smap + lane * 64);
+                        uint4* row_map = reinterpret_cast<uint4*>(smap + lane * TR_MSTR);
 #pragma unroll
-                        for (int k = 0; k < 4; k++) row_map[k] = make_uint4(0, 0, 0, 0);
+                        for (int k = 0; k < TR_MSTR / 16; k++) row_map[k] = make_uint4(0, 0, 0, 0);
                         const uint32_t ei = ebase + (uint32_t)lane;
                         const uint32_t v = ei < E_hi ? a.vals[ei] : 0x80000000u;
                         if (!(v >> 31)) {
@@ -783,8 +787,8 @@ __global__ __launch_bounds__(WAVE) void k_tile_rows(TileArgs a) {
                             for (int k = 0; k < TR_SLOTS; k++) {
                                 float area, height;
                                 dda_increment(d, area, height);
-                                inc[lane * TR_SLOTS + k] = make_float2(area, height);
-                                smap[lane * 64 + (((d.y & 7) << 3) | (d.x & 7))] = (uint8_t)(k + 1);
+                                inc[lane * TR_SSTR + k] = make_float2(area, height);
+                                smap[lane * TR_MSTR + (((d.y & 7) << 3) | (d.x & 7))] = (uint8_t)(k + 1);
                                 if (dda_step(d) || div8(d.x) != rtx || div8(d.y) != rty) {
                                     ended = true;
                                     break;
@@ -800,11 +804,11 @@ __global__ __launch_bounds__(WAVE) void k_tile_rows(TileArgs a) {
                 const uint32_t m = min(min(8u, g_end - e), (uint32_t)TR_BLOCK - jr);
                 uint32_t k8[8];
 #pragma unroll
-                for (int i = 0; i < 8; i++) k8[i] = (uint32_t)i < m ? smap[(jr + i) * 64 + lane] : 0u;
+                for (int i = 0; i < 8; i++) k8[i] = (uint32_t)i < m ? smap[(jr + i) * TR_MSTR + lane] : 0u;
 #pragma unroll
                 for (int i = 0; i < 8; i++) {
                     if (k8[i]) {
-                        const float2 v = inc[(jr + i) * TR_SLOTS + k8[i] - 1];
+                        const float2 v = inc[(jr + i) * TR_SSTR + k8[i] - 1];
                         pa = pa + v.x;
                         ph = ph + v.y;
                     }
@@ -847,7 +851,8 @@ __global__ __launch_bounds__(WAVE) void k_tile_rows(TileArgs a) {
 // the tile and span sets, which are integer, are exact.  One workgroup per tile
 // row walks the row's windows left to right, carrying the winding, the last
 // tile and the per-pixel-row height carry across them.
-constexpr int UW = 64;  // tiles per window
+constexpr int UW = 64;   // tiles per window
+constexpr int UWS = 65;  // LDS words per tile's sums (one of padding: lanes in different tiles hit different banks)
 
 struct LoadNWin {
     const PathGrid* g;
@@ -1009,8 +1014,8 @@ constexpr int RT_U = 4;  // entries per thread whose loads are issued together
 
 template <bool EMIT>
 struct RowLds {
-    float acc_a[EMIT ? UW * 64 : 1];
-    float acc_h[EMIT ? UW * 64 : 1];
+    float acc_a[EMIT ? UW * UWS : 1];  // tile t, pixel p at t * UWS + p
+    float acc_h[EMIT ? UW * UWS : 1];
     int32_t tsign[UW];
     uint32_t occ[UW / 32];
     uint8_t tlist[UW];  // the window's occupied tiles in x order
@@ -1120,8 +1125,9 @@ __global__ __launch_bounds__(RT_NT) void k_row_tiles(RowTileArgs a) {
                 // zero the occupied tiles' sums, then replay the window's runs into them
                 for (int k = tid; k < UW * 64; k += RT_NT) {
                     if ((s.occ[k >> 11] >> ((k >> 6) & 31)) & 1u) {
-                        s.acc_a[k] = 0.0f;
-                        s.acc_h[k] = 0.0f;
+                        const int o = (k >> 6) * UWS + (k & 63);
+                        s.acc_a[o] = 0.0f;
+                        s.acc_h[o] = 0.0f;
                     }
                 }
                 __syncthreads();
@@ -1171,7 +1177,7 @@ __global__ __launch_bounds__(RT_NT) void k_row_tiles(RowTileArgs a) {
                     for (int k = 0; k < 16; k++) {
                         float area, height;
                         dda_increment(d, area, height);
-                        const uint32_t px = x * 64u + (uint32_t)(((d.y & 7) << 3) | (d.x & 7));
+                        const uint32_t px = x * (uint32_t)UWS + (uint32_t)(((d.y & 7) << 3) | (d.x & 7));
                         atomicAdd(&s.acc_a[px], area);
                         atomicAdd(&s.acc_h[px], height);
                         if (dda_step(d) || div8(d.x) != rtx || div8(d.y) != rty) {
@@ -1189,7 +1195,7 @@ __global__ __launch_bounds__(RT_NT) void k_row_tiles(RowTileArgs a) {
                 // quantise adds the two.  (Regrouping the row's f32 sums this way
                 // moves a pixel's carry by a few ulps: within the 1-LSB contract.)
                 for (uint32_t item = (uint32_t)tid; item < nocc * 8u; item += RT_NT) {
-                    const int base = (int)s.tlist[item >> 3] * 64 + (int)(item & 7u) * 8;
+                    const int base = (int)s.tlist[item >> 3] * UWS + (int)(item & 7u) * 8;
                     float acc = 0.0f;
 #pragma unroll
                     for (int k = 0; k < 8; k++) {
@@ -1212,7 +1218,7 @@ __global__ __launch_bounds__(RT_NT) void k_row_tiles(RowTileArgs a) {
                     // item -> (the occupied tile of rank item / 16, 4 pixels)
                     const uint32_t rk = item >> 4, quad = item & 15u;
                     const int t = s.tlist[rk];
-                    const int base = t * 64 + (int)quad * 4;
+                    const int base = t * UWS + (int)quad * 4;
                     uint32_t word = 0;
                     const float cin = s.rtot[rk * 8 + (quad >> 1)];  // pixel row quad / 2
 #pragma unroll

@@ -23,7 +23,7 @@ def test_header_declares_the_bound_symbols():
 
 
 def test_library_exports_every_declared_symbol():
-    lib = ctypes.CDLL(_lib.LIB_PATH)
+    lib = _lib.load()
     missing = [n for n in declared_functions() if not hasattr(lib, n)]
     assert not missing, missing
 
