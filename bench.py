@@ -295,6 +295,8 @@ def main():
                     help="dense kernel launch shape (workgroups per CU, waves per workgroup) instead of autotuning")
     ap.add_argument("--dense-small", type=int, default=None, metavar="G",
                     help="workgroups per CU of the small-grid dense instance (0: off) instead of autotuning")
+    ap.add_argument("--dense-large", type=int, default=None, choices=[0, 1],
+                    help="the 256-tile dense instance off (0) or on (1) instead of autotuning")
     ap.add_argument("--no-dense", action="store_true", help="the ordered batched kernel alone (bit-exact sums)")
     ap.add_argument("--no-graph", action="store_true", help="launch the batched kernel directly, not from a graph")
     ap.add_argument("--adjoint", action="store_true",
@@ -370,6 +372,9 @@ def main():
         if args.dense_small is not None:
             from ochre_amd.device import set_dense_small
             set_dense_small(ctx, args.dense_small)
+        if args.dense_large is not None:
+            from ochre_amd.device import set_dense_large
+            set_dense_large(ctx, bool(args.dense_large))
         if args.params or (args.dense and not args.no_dense):
             w, g, tc = (int(x) for x in (args.params or "8,1,32").split(","))
             set_batched_params(ctx, w, g, tc, graph=not args.no_graph)

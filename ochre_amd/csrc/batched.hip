@@ -752,6 +752,7 @@ __global__ __launch_bounds__(WAVE) void k_fused(FusedIn in, FusedOut out, FusedC
     __shared__ LdsState<TC> s;
     uint8_t* sc = scratch + (size_t)blockIdx.x * F_SCRATCH;
     const int64_t n = in.ids ? (int64_t)*in.n_ids : in.n_paths;
+    if (n == 0) return;  // nothing handed on (the common case behind the dense kernels)
     if (in.ids && n > (int64_t)in.defer_above) {
         for (int64_t q = (int64_t)blockIdx.x * WAVE + threadIdx.x; q < n; q += (int64_t)gridDim.x * WAVE)
             deferred[atomicAdd(&ctl->deferred, 1u)] = in.ids[q];
