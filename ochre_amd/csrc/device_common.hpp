@@ -201,6 +201,23 @@ __device__ inline void dda_init(Dda& d, V2 last, V2 point) {
 // The line's point at parameter t: (1 - t) * last + t * point (rasterizer.rs:100-101).
 __device__ inline V2 dda_point(const Dda& d, float t) { return add(smul(1.0f - t, d.last), smul(t, d.point)); }
 
+// The increment at the current pixel with its start point p0 carried from
+// the previous step (rasterizer.rs:98-106), p0 left at its end point for the
+// next.  With finite end points row_t1 and col_t1 are never NaN (each is
+// clamped by fminf), so the step's t0 is exactly the previous increment's
+// t1 — up to the sign of a zero t, which no sum or alpha can see; for other
+// lines the start point is recomputed from t0.
+__device__ inline void dda_increment_carried(const Dda& d, V2& p0, bool finite, float& area, float& height) {
+    if (!finite) p0 = dda_point(d, d.t0);
+    const V2 p1 = dda_point(d, fminf(d.row_t1, d.col_t1));
+    height = p1.y - p0.y;
+    const float right = (float)wrap16(d.x + 1);
+    area = 0.5f * height * ((right - p0.x) + (right - p1.x));
+    p0 = p1;
+}
+
+__device__ inline bool line_finite(V2 a, V2 b) { return isfinite(a.x) && isfinite(a.y) && isfinite(b.x) && isfinite(b.y); }
+
 // The increment pushed at the current pixel (rasterizer.rs:98-106).
 __device__ inline void dda_increment(const Dda& d, float& area, float& height) {
     float t1 = fminf(d.row_t1, d.col_t1);

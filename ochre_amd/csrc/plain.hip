@@ -783,10 +783,12 @@ __global__ __launch_bounds__(WAVE) void k_tile_rows(TileArgs a) {
                                 dda_step(d);
                             }
                             const int16_t rtx = div8(d.x), rty = div8(d.y);
+                            const bool finite = line_finite(last, point);
+                            V2 p0 = dda_point(d, d.t0);
                             bool ended = false;
                             for (int k = 0; k < TR_SLOTS; k++) {
                                 float area, height;
-                                dda_increment(d, area, height);
+                                dda_increment_carried(d, p0, finite, area, height);
                                 inc[lane * TR_SSTR + k] = make_float2(area, height);
                                 smap[lane * TR_MSTR + (((d.y & 7) << 3) | (d.x & 7))] = (uint8_t)(k + 1);
                                 if (dda_step(d) || div8(d.x) != rtx || div8(d.y) != rty) {
@@ -1173,10 +1175,12 @@ __global__ __launch_bounds__(RT_NT) void k_row_tiles(RowTileArgs a) {
                         dda_step(d);
                     }
                     const int16_t rtx = div8(d.x), rty = div8(d.y);
+                    const bool finite = line_finite(last, point);
+                    V2 p0 = dda_point(d, d.t0);
                     bool ended = false;
                     for (int k = 0; k < 16; k++) {
                         float area, height;
-                        dda_increment(d, area, height);
+                        dda_increment_carried(d, p0, finite, area, height);
                         const uint32_t px = x * (uint32_t)UWS + (uint32_t)(((d.y & 7) << 3) | (d.x & 7));
                         atomicAdd(&s.acc_a[px], area);
                         atomicAdd(&s.acc_h[px], height);
