@@ -690,7 +690,7 @@ __global__ __launch_bounds__(WAVE) void k_tile_counts(TileArgs a) {
 // is written where the previous tile of the row is more than one tile to the
 // left and the winding after it is non-zero (rasterizer.rs:252-264).
 constexpr int TR_SLOTS = 16;  // increments per run slot
-constexpr int TR_BLOCK = 32;  // runs replayed per block (6 KB of LDS per wave)
+constexpr int TR_BLOCK = 32;  // runs replayed per block (6 KB of LDS per wave; 16 and 64 measured slower)
 // (Padding these rows against the lane-strided bank conflicts of the replay
 // was measured slower: the replay is a small part of the kernel.)
 constexpr int TR_SSTR = TR_SLOTS;  // float2 per run slot row
