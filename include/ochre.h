@@ -219,7 +219,8 @@ int ochre_context_get_batched_params(ochre_context* ctx, int* waves_per_cu, int*
  * order and an alpha can differ from the ordered result by 1 LSB (tile and
  * span sets are exact).  Off: every path goes through the ordered kernel,
  * whose sums follow emission order.  groups_per_cu 1-8 resident workgroups,
- * waves_per_group 2 or 4 (waves sharing one path). */
+ * waves_per_group 2, 4 or 8 (waves sharing one path; with 8, at most 2
+ * workgroups per CU are resident). */
 int ochre_context_set_dense_params(ochre_context* ctx, int enabled, int groups_per_cu, int waves_per_group);
 int ochre_context_get_dense_params(ochre_context* ctx, int* enabled, int* groups_per_cu, int* waves_per_group);
 /* A small-grid dense instance (tile grids of at most 32 tiles, 2 waves and
@@ -270,7 +271,7 @@ int ochre_coverage_adjoint(ochre_context* ctx, const ochre_path_batch* in, const
                            float* loss, float* d_points, float* d_transforms, void* stream);
 
 /* Autotuner: with the dense kernel on, times every (workgroups per CU
- * {1,2,3,4}, waves per workgroup {2,4}) of the 64-tile dense launch, then
+ * {1,2,3,4}, waves per workgroup {2,4,8}) of the 64-tile dense launch, then
  * the small-grid instance {off, 4, 6, 8 workgroups per CU}, the 256-tile
  * instance {off, on}, then waves per CU {1,2,4,8} of the ordered kernel
  * behind them; with it off,

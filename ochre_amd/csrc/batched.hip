@@ -863,7 +863,7 @@ int run_batched(Context* ctx, const SegInput& in, OutputDesc& out, hipStream_t s
     const uint32_t g = (uint32_t)std::max(1, gran);
     const bool tc16 = tiles_per_chunk <= 16;
     const bool dense = ctx->dense;
-    const int dnw = ctx->dense_waves == 2 ? 2 : 4;
+    const int dnw = ctx->dense_waves == 2 ? 2 : (ctx->dense_waves == 8 ? 8 : 4);
     const int n_groups = std::max(1, cus * std::max(1, ctx->dense_groups_per_cu));
     const int n_small = dense ? cus * std::max(0, ctx->dense_small_groups) : 0;
     const int n_large = (dense && ctx->dense_large) ? cus : 0;
@@ -917,6 +917,9 @@ int run_batched(Context* ctx, const SegInput& in, OutputDesc& out, hipStream_t s
                                    lists[i], counts[i], dscratch);
             else if (dnw == 2)
                 hipLaunchKernelGGL((k_dense<2, 64>), dim3(n_groups), dim3(2 * WAVE), 0, st, f, fo, ctl, queues[i],
+                                   lists[i], counts[i], dscratch);
+            else if (dnw == 8)
+                hipLaunchKernelGGL((k_dense<8, 64>), dim3(n_groups), dim3(8 * WAVE), 0, st, f, fo, ctl, queues[i],
                                    lists[i], counts[i], dscratch);
             else
                 hipLaunchKernelGGL((k_dense<4, 64>), dim3(n_groups), dim3(4 * WAVE), 0, st, f, fo, ctl, queues[i],

@@ -248,7 +248,8 @@ int ochre_context_get_batched_params(ochre_context* ctx, int* waves_per_cu, int*
 }
 
 int ochre_context_set_dense_params(ochre_context* ctx, int enabled, int groups_per_cu, int waves_per_group) {
-    if (!ctx || groups_per_cu < 1 || groups_per_cu > 8 || (waves_per_group != 2 && waves_per_group != 4))
+    if (!ctx || groups_per_cu < 1 || groups_per_cu > 8 ||
+        (waves_per_group != 2 && waves_per_group != 4 && waves_per_group != 8))
         return OCHRE_ERR_INVALID_ARG;
     std::lock_guard<std::mutex> lk(ctx->mu);
     ctx->c.dense = enabled != 0;
@@ -338,7 +339,7 @@ int ochre_autotune_batched(ochre_context* ctx, const ochre_path_batch* in, ochre
     static const int grans[] = {1, 2, 4, 8};
     static const int tcs[] = {16, 32};
     static const int dgroups[] = {1, 2, 3, 4};
-    static const int dwaves[] = {2, 4};
+    static const int dwaves[] = {2, 4, 8};
     ochre_batched_tuning best{ctx->c.batched_waves_per_cu, ctx->c.batched_gran, ctx->c.batched_tc, 1e30f, 0, 0, 0, 0};
     int best_dg = ctx->c.dense_groups_per_cu, best_dw = ctx->c.dense_waves;
     int k = 0;
@@ -364,6 +365,7 @@ int ochre_autotune_batched(ochre_context* ctx, const ochre_path_batch* in, ochre
         // encoded as (groups per CU, waves per group) in the results
         for (int dg : dgroups)
             for (int dw : dwaves) {
+                if (dw == 8 && dg > 2) continue;  // 16 waves of 128 registers fill a CU
                 ctx->c.dense_groups_per_cu = dg;
                 ctx->c.dense_waves = dw;
                 float ms = 0;

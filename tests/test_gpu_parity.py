@@ -234,7 +234,8 @@ def test_batched_params_and_graph_replays(gpu_ctx):
     configs = [(0, 0, 0, 0, 0, 2, 1, 32, 1), (0, 0, 0, 0, 0, 6, 1, 16, 1), (0, 0, 0, 0, 0, 8, 4, 32, 1),
                (0, 0, 0, 0, 0, 4, 16, 16, 0), (0, 0, 0, 0, 0, 1, 2, 32, 0), (1, 4, 4, 8, 1, 6, 2, 32, 1),
                (1, 2, 2, 0, 0, 6, 2, 32, 1), (1, 1, 4, 4, 1, 2, 1, 16, 0), (1, 3, 4, 6, 0, 8, 1, 32, 1),
-               (1, 4, 4, 0, 1, 1, 1, 32, 1), (1, 4, 4, 0, 0, 1, 2, 32, 1)]
+               (1, 4, 4, 0, 1, 1, 1, 32, 1), (1, 4, 4, 0, 0, 1, 2, 32, 1), (1, 2, 8, 0, 1, 2, 2, 32, 1),
+               (1, 1, 8, 4, 0, 1, 1, 32, 0)]
     try:
         for dense, dg, dw, sg, lg, waves, gran, tc, graph in configs:
             assert lib.ochre_context_set_dense_params(gpu_ctx.h, dense, max(dg, 1), dw or 4) == 0
