@@ -421,6 +421,14 @@ def main():
                 t, n = ctx.timing(k)
                 if n:
                     kernels[k] = t / n  # the last two: the plain pipeline over the handed-on paths
+            stats = {}
+            for k in ("lines", "entries", "runs", "rows", "windows", "windows_used"):
+                t, n = ctx.timing("pass." + k)
+                if n:
+                    stats[k] = int(t) // (args.steps + 1)
+            if stats:
+                stats["passes"] = n // (args.steps + 1)
+                log(f"handed-on paths, per call: {stats}")
             ctx.set_timing(False)
             set_batched_params(ctx, bp["waves_per_cu"], bp["paths_per_pop"], bp["tiles_per_chunk"],
                                graph=bool(bp["graph"]))

@@ -295,7 +295,11 @@ int ochre_autotune_batched(ochre_context* ctx, const ochre_path_batch* in, ochre
 
 /* Kernel-time accounting for measurement: when enabled, the pipeline
  * records HIP events around its dominant kernel; ochre_context_timing
- * returns the summed milliseconds and launch count since the last reset. */
+ * returns the summed milliseconds and launch count since the last reset.
+ * The unordered tile stage of the plain pipeline also records pass
+ * statistics under "pass.lines", "pass.entries", "pass.runs", "pass.rows",
+ * "pass.windows" and "pass.windows_used" (the sum over passes in total_ms,
+ * the pass count in launches). */
 int ochre_context_set_timing(ochre_context* ctx, int enabled);
 int ochre_context_timing(ochre_context* ctx, const char* kernel, double* total_ms, int64_t* launches);
 

@@ -1511,6 +1511,21 @@ static int run_plain_pass(Context* ctx, const SegInput& in, OutputDesc& out, hip
         out.n_spans = S;
         if (T > out.tile_capacity || S > out.span_capacity) return -4;
         if (ctx->timing) {
+            // pass statistics beside the kernel times: lines, sort entries,
+            // runs, tile rows, windows and the windows holding entries
+            std::vector<uint32_t> bc((size_t)NB);
+            if (NB > 0) HIPCHK(hipMemcpyAsync(bc.data(), bcnt, (size_t)NB * 4, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            int64_t used = 0;
+            for (uint32_t c : bc) used += c != 0;
+            const std::pair<const char*, int64_t> st[] = {{"pass.lines", L},   {"pass.entries", R},
+                                                         {"pass.runs", RUNS},  {"pass.rows", ROWS},
+                                                         {"pass.windows", NB}, {"pass.windows_used", used}};
+            for (const auto& kv : st) {
+                auto& t = ctx->timings[kv.first];
+                t.first += (double)kv.second;
+                t.second += 1;
+            }
             if (!ctx->ev[0]) HIPCHK(hipEventCreate(&ctx->ev[0]));
             if (!ctx->ev[1]) HIPCHK(hipEventCreate(&ctx->ev[1]));
             HIPCHK(hipEventRecord(ctx->ev[0], s));
