@@ -878,33 +878,70 @@ __device__ inline uint32_t entry_bucket(uint32_t key, uint32_t xbits, const uint
     return row_wbase[key >> xbits] + ((key & ((1u << xbits) - 1u)) / UW);
 }
 
-// Bucketing: each workgroup takes BK_ITEMS entries per thread and first
-// counts them per bucket in an LDS hash table (open addressing, room for every
-// entry), so that a bucket gets one global atomic per workgroup rather than
-// one per entry: the entries of a workgroup are consecutive in emission order
-// (the runs and crossings of a few neighbouring lines) and fall into few
-// buckets.
+// Bucketing: each workgroup takes BK_ITEMS entries per thread, consecutive
+// entries on consecutive lanes.  The entries are in emission order (the runs
+// and crossings of neighbouring lines along a contour), so consecutive ones
+// mostly share a bucket: in each wave only the first lane of a run of equal
+// buckets inserts the run's length into an LDS hash table (open addressing,
+// room for every entry), and a bucket gets one global atomic per workgroup.
 constexpr int BK_NT = 256;
-constexpr int BK_ITEMS = 2;
-constexpr int BK_HS = 1024;  // >= BK_NT * BK_ITEMS
+constexpr int BK_ITEMS = 8;
+constexpr int BK_HBITS = 11;
+constexpr int BK_HS = 1 << BK_HBITS;  // == BK_NT * BK_ITEMS
 constexpr uint32_t BK_EMPTY = 0xffffffffu;
+static_assert(BK_HS >= BK_NT * BK_ITEMS, "the bucket hash must hold every entry of a workgroup");
 
 struct BucketLds {
     uint32_t key[BK_HS];
     uint32_t cnt[BK_HS];
-    uint32_t base[BK_HS];
+    uint32_t aux[BK_HS];  // histogram: the bucket's tile row; scatter: the workgroup's first slot in the bucket
+    int32_t sgn[BK_HS];   // histogram: the bucket's crossing sum
 };
 
-// Inserts bucket b; returns its slot and the entry's rank within the block.
-__device__ inline uint32_t bucket_insert(BucketLds& h, uint32_t b, uint32_t* rank) {
-    uint32_t slot = (b * 2654435761u) >> (32 - 10);  // BK_HS = 2^10
+__device__ inline uint64_t lanemask_le() { return lanemask_lt() | (1ull << (threadIdx.x & (WAVE - 1))); }
+
+// Inserts n entries of bucket b; returns the bucket's slot and the first
+// entry's rank within the workgroup.  Terminates: the table has room for
+// every entry of the workgroup, hence for every distinct bucket.
+__device__ inline uint32_t bucket_insert(BucketLds& h, uint32_t b, uint32_t n, uint32_t* rank) {
+    uint32_t slot = (b * 2654435761u) >> (32 - BK_HBITS);
     for (;;) {
         const uint32_t old = atomicCAS(&h.key[slot], BK_EMPTY, b);
         if (old == BK_EMPTY || old == b) {
-            *rank = atomicAdd(&h.cnt[slot], 1u);
+            *rank = atomicAdd(&h.cnt[slot], n);
             return slot;
         }
         slot = (slot + 1) & (BK_HS - 1);
+    }
+}
+
+// One wave's 64 consecutive entries (the valid ones a prefix of the lanes):
+// the slot of each entry's bucket and its rank within the workgroup.
+__device__ inline void bucket_wave(BucketLds& h, bool valid, uint32_t b, uint32_t* slot, uint32_t* rank,
+                                   uint32_t* head_row, uint32_t row) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    const uint32_t prev = __shfl_up(b, 1, WAVE);
+    const bool head = valid && (lane == 0 || prev != b);
+    const uint64_t hm = __ballot(head);
+    const int nvalid = __popcll(__ballot(valid));
+    uint32_t sl = 0, rk = 0;
+    if (head) {
+        const uint64_t above = hm & ~lanemask_le();
+        const int next = above ? __ffsll((unsigned long long)above) - 1 : nvalid;
+        sl = bucket_insert(h, b, (uint32_t)(next - lane), &rk);
+        if (head_row) head_row[sl] = row;
+    }
+    const int hl = valid ? 63 - __clzll((long long)(hm & lanemask_le())) : lane;
+    *slot = __shfl(sl, hl, WAVE);
+    *rank = __shfl(rk, hl, WAVE) + (uint32_t)(lane - hl);
+}
+
+template <bool HIST>
+__device__ inline void bucket_init(BucketLds& h) {
+    for (int k = threadIdx.x; k < BK_HS; k += BK_NT) {
+        h.key[k] = BK_EMPTY;
+        h.cnt[k] = 0;
+        if (HIST) h.sgn[k] = 0;
     }
 }
 
@@ -913,28 +950,34 @@ __global__ __launch_bounds__(BK_NT) void k_bucket_hist(const uint32_t* keys, con
                                                        uint32_t xbits, const uint32_t* row_wbase, uint32_t* bcnt,
                                                        int32_t* row_sign) {
     __shared__ BucketLds h;
-    for (int k = threadIdx.x; k < BK_HS; k += BK_NT) {
-        h.key[k] = BK_EMPTY;
-        h.cnt[k] = 0;
-    }
+    bucket_init<true>(h);
     __syncthreads();
     const int64_t e0 = (int64_t)blockIdx.x * BK_NT * BK_ITEMS + threadIdx.x;
+    uint32_t kk[BK_ITEMS], vv[BK_ITEMS];
 #pragma unroll
     for (int u = 0; u < BK_ITEMS; u++) {
         const int64_t e = e0 + (int64_t)u * BK_NT;
-        if (e < R) {
-            const uint32_t key = keys[e], v = vals[e];
-            uint32_t rank;
-            (void)bucket_insert(h, entry_bucket(key, xbits, row_wbase), &rank);
-            if (v >> 31) {
-                const int sg = (int)(int8_t)(v & 0xffu);
-                if (sg) atomicAdd(&row_sign[key >> xbits], sg);
-            }
+        kk[u] = e < R ? keys[e] : 0u;
+        vv[u] = e < R ? vals[e] : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < BK_ITEMS; u++) {
+        const bool valid = e0 + (int64_t)u * BK_NT < R;
+        const uint32_t row = kk[u] >> xbits;
+        const uint32_t b = valid ? entry_bucket(kk[u], xbits, row_wbase) : BK_EMPTY;
+        uint32_t slot, rank;
+        bucket_wave(h, valid, b, &slot, &rank, h.aux, row);
+        if (valid && (vv[u] >> 31)) {
+            const int sg = (int)(int8_t)(vv[u] & 0xffu);
+            if (sg) atomicAdd(&h.sgn[slot], sg);
         }
     }
     __syncthreads();
-    for (int k = threadIdx.x; k < BK_HS; k += BK_NT)
-        if (h.key[k] != BK_EMPTY) atomicAdd(&bcnt[h.key[k]], h.cnt[k]);
+    for (int k = threadIdx.x; k < BK_HS; k += BK_NT) {
+        if (h.key[k] == BK_EMPTY) continue;
+        atomicAdd(&bcnt[h.key[k]], h.cnt[k]);
+        if (h.sgn[k]) atomicAdd(&row_sign[h.aux[k]], h.sgn[k]);
+    }
 }
 
 // Scatter into the buckets: one cursor atomic per bucket per workgroup.
@@ -942,31 +985,31 @@ __global__ __launch_bounds__(BK_NT) void k_bucket_scatter(const uint32_t* keys, 
                                                           uint32_t xbits, const uint32_t* row_wbase, uint32_t* bcur,
                                                           uint32_t* okeys, uint32_t* ovals) {
     __shared__ BucketLds h;
-    for (int k = threadIdx.x; k < BK_HS; k += BK_NT) {
-        h.key[k] = BK_EMPTY;
-        h.cnt[k] = 0;
-    }
+    bucket_init<false>(h);
     __syncthreads();
     const int64_t e0 = (int64_t)blockIdx.x * BK_NT * BK_ITEMS + threadIdx.x;
     uint32_t kk[BK_ITEMS], vv[BK_ITEMS], slot[BK_ITEMS], rank[BK_ITEMS];
 #pragma unroll
     for (int u = 0; u < BK_ITEMS; u++) {
         const int64_t e = e0 + (int64_t)u * BK_NT;
-        slot[u] = BK_EMPTY;
-        if (e < R) {
-            kk[u] = keys[e];
-            vv[u] = vals[e];
-            slot[u] = bucket_insert(h, entry_bucket(kk[u], xbits, row_wbase), &rank[u]);
-        }
+        kk[u] = e < R ? keys[e] : 0u;
+        vv[u] = e < R ? vals[e] : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < BK_ITEMS; u++) {
+        const bool valid = e0 + (int64_t)u * BK_NT < R;
+        const uint32_t b = valid ? entry_bucket(kk[u], xbits, row_wbase) : BK_EMPTY;
+        bucket_wave(h, valid, b, &slot[u], &rank[u], nullptr, 0u);
+        if (!valid) slot[u] = BK_EMPTY;
     }
     __syncthreads();
     for (int k = threadIdx.x; k < BK_HS; k += BK_NT)
-        if (h.key[k] != BK_EMPTY) h.base[k] = atomicAdd(&bcur[h.key[k]], h.cnt[k]);
+        if (h.key[k] != BK_EMPTY) h.aux[k] = atomicAdd(&bcur[h.key[k]], h.cnt[k]);
     __syncthreads();
 #pragma unroll
     for (int u = 0; u < BK_ITEMS; u++) {
         if (slot[u] != BK_EMPTY) {
-            const uint32_t o = h.base[slot[u]] + rank[u];
+            const uint32_t o = h.aux[slot[u]] + rank[u];
             okeys[o] = kk[u];
             ovals[o] = vv[u];
         }
@@ -1016,8 +1059,8 @@ constexpr int RT_U = 4;  // entries per thread whose loads are issued together
 
 template <bool EMIT>
 struct RowLds {
-    float acc_a[EMIT ? UW * UWS : 1];  // tile t, pixel p at t * UWS + p
-    float acc_h[EMIT ? UW * UWS : 1];
+    float acc_a[EMIT ? UW * UWS : 4];  // tile t, pixel p at t * UWS + p
+    float acc_h[EMIT ? UW * UWS : 4];
     int32_t tsign[UW];
     uint32_t occ[UW / 32];
     uint8_t tlist[UW];  // the window's occupied tiles in x order
@@ -1025,9 +1068,44 @@ struct RowLds {
     uint32_t nt, ns, nocc;
 };
 
+// Replays a run (x: its tile within the window) into the window's LDS sums
+// with float atomics (arrival order).
+template <bool EMIT>
+__device__ inline void rt_replay_run(RowLds<EMIT>& s, uint32_t x, const RunRec& rr, const float4& L, Counters* cnt) {
+    Dda d;
+    const V2 last = mk(L.x, L.y), point = mk(L.z, L.w);
+    if (rr.line >> 31) {
+        dda_init(d, last, point);
+    } else {
+        dda_consts(d, last, point);
+        d.x = (int16_t)(rr.xy & 0xffffu);
+        d.y = (int16_t)(rr.xy >> 16);
+        d.row_t1 = rr.row_t1;
+        d.col_t1 = rr.col_t1;
+        d.t0 = 0.0f;
+        dda_step(d);
+    }
+    const int16_t rtx = div8(d.x), rty = div8(d.y);
+    const bool finite = line_finite(last, point);
+    V2 p0 = dda_point(d, d.t0);
+    bool ended = false;
+    for (int k = 0; k < 16; k++) {
+        float area, height;
+        dda_increment_carried(d, p0, finite, area, height);
+        const uint32_t px = x * (uint32_t)UWS + (uint32_t)(((d.y & 7) << 3) | (d.x & 7));
+        atomicAdd(&s.acc_a[px], area);
+        atomicAdd(&s.acc_h[px], height);
+        if (dda_step(d) || div8(d.x) != rtx || div8(d.y) != rty) {
+            ended = true;
+            break;
+        }
+    }
+    if (!ended) atomicOr(&cnt->error, (uint32_t)ERR_REPLAY);
+}
+
 template <bool EMIT>
 __global__ __launch_bounds__(RT_NT) void k_row_tiles(RowTileArgs a) {
-    __shared__ RowLds<EMIT> s;
+    __shared__ __align__(16) RowLds<EMIT> s;
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), wid = tid / WAVE;
     const uint32_t xmask = (1u << a.xbits) - 1u;
     const uint64_t lt = lanemask_lt();
@@ -1056,26 +1134,64 @@ __global__ __launch_bounds__(RT_NT) void k_row_tiles(RowTileArgs a) {
             const int32_t x0 = (int32_t)(wi * UW);
             if (tid < UW) s.tsign[tid] = 0;
             if (tid < UW / 32) s.occ[tid] = 0;
+            if constexpr (EMIT) {
+                // all the window's sums start at zero (the runs add into them
+                // before the occupied tiles are known)
+                float4* za = reinterpret_cast<float4*>(s.acc_a);
+                float4* zh = reinterpret_cast<float4*>(s.acc_h);
+                for (int k = tid; k < UW * UWS / 4; k += RT_NT) {
+                    za[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    zh[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                }
+            }
             __syncthreads();
-            // occupancy and crossing sums from the keys alone, RT_U entries
-            // per thread at a time (their loads in flight together)
+            // one pass over the window's entries, RT_U per thread at a time
+            // (each level of their loads in flight together): occupancy and
+            // crossing sums from the keys, and (emit pass) the runs replayed
             for (uint32_t e0 = eb + (uint32_t)tid; e0 < ee; e0 += RT_U * RT_NT) {
                 uint32_t kk[RT_U], vv[RT_U];
 #pragma unroll
                 for (int u = 0; u < RT_U; u++) {
                     const uint32_t e = e0 + (uint32_t)(u * RT_NT);
                     kk[u] = e < ee ? a.keys[e] : 0u;
-                    vv[u] = e < ee ? a.vals[e] : 0u;
+                    vv[u] = e < ee ? a.vals[e] : 0x80000000u;
                 }
+                if constexpr (EMIT) {
+                    RunRec rrs[RT_U];
 #pragma unroll
-                for (int u = 0; u < RT_U; u++) {
-                    if (e0 + (uint32_t)(u * RT_NT) >= ee) break;
-                    const uint32_t x = (kk[u] & xmask) - (uint32_t)x0;
-                    if (vv[u] >> 31) {
-                        const int sg = (int)(int8_t)(vv[u] & 0xffu);
-                        if (sg) atomicAdd(&s.tsign[x], sg);
-                    } else {
-                        atomicOr(&s.occ[x >> 5], 1u << (x & 31));
+                    for (int u = 0; u < RT_U; u++) {
+                        rrs[u].line = 0;
+                        rrs[u].xy = 0;
+                        rrs[u].row_t1 = rrs[u].col_t1 = 0.0f;
+                        if (!(vv[u] >> 31)) rrs[u] = a.recs[vv[u]];
+                    }
+                    float4 Ls[RT_U];
+#pragma unroll
+                    for (int u = 0; u < RT_U; u++)
+                        Ls[u] = !(vv[u] >> 31) ? a.lines[rrs[u].line & 0x7fffffffu] : make_float4(0, 0, 0, 0);
+#pragma unroll
+                    for (int u = 0; u < RT_U; u++) {
+                        if (e0 + (uint32_t)(u * RT_NT) >= ee) break;
+                        const uint32_t x = (kk[u] & xmask) - (uint32_t)x0;
+                        if (vv[u] >> 31) {
+                            const int sg = (int)(int8_t)(vv[u] & 0xffu);
+                            if (sg) atomicAdd(&s.tsign[x], sg);
+                        } else {
+                            atomicOr(&s.occ[x >> 5], 1u << (x & 31));
+                            rt_replay_run(s, x, rrs[u], Ls[u], a.cnt);
+                        }
+                    }
+                } else {
+#pragma unroll
+                    for (int u = 0; u < RT_U; u++) {
+                        if (e0 + (uint32_t)(u * RT_NT) >= ee) break;
+                        const uint32_t x = (kk[u] & xmask) - (uint32_t)x0;
+                        if (vv[u] >> 31) {
+                            const int sg = (int)(int8_t)(vv[u] & 0xffu);
+                            if (sg) atomicAdd(&s.tsign[x], sg);
+                        } else {
+                            atomicOr(&s.occ[x >> 5], 1u << (x & 31));
+                        }
                     }
                 }
             }
@@ -1121,78 +1237,9 @@ __global__ __launch_bounds__(RT_NT) void k_row_tiles(RowTileArgs a) {
                 nt += (uint32_t)__popcll(om);
                 ns += (uint32_t)__popcll(sm);
             }
-            if (EMIT) {
+            if constexpr (EMIT) {
                 __syncthreads();
                 const uint32_t nocc = s.nocc, nt_w = s.nt;
-                // zero the occupied tiles' sums, then replay the window's runs into them
-                for (int k = tid; k < UW * 64; k += RT_NT) {
-                    if ((s.occ[k >> 11] >> ((k >> 6) & 31)) & 1u) {
-                        const int o = (k >> 6) * UWS + (k & 63);
-                        s.acc_a[o] = 0.0f;
-                        s.acc_h[o] = 0.0f;
-                    }
-                }
-                __syncthreads();
-                for (uint32_t e0 = eb + (uint32_t)tid; e0 < ee; e0 += RT_U * RT_NT) {
-                  // the RT_U entries' values, run records and lines are loaded
-                  // level by level, each level's loads in flight together
-                  uint32_t vv[RT_U], xx[RT_U];
-                  RunRec rrs[RT_U];
-                  float4 Ls[RT_U];
-#pragma unroll
-                  for (int u = 0; u < RT_U; u++) {
-                      const uint32_t e = e0 + (uint32_t)(u * RT_NT);
-                      vv[u] = e < ee ? a.vals[e] : 0x80000000u;
-                      xx[u] = e < ee ? (a.keys[e] & xmask) - (uint32_t)x0 : 0u;
-                  }
-#pragma unroll
-                  for (int u = 0; u < RT_U; u++) {
-                      rrs[u].line = 0;
-                      rrs[u].xy = 0;
-                      rrs[u].row_t1 = rrs[u].col_t1 = 0.0f;
-                      if (!(vv[u] >> 31)) rrs[u] = a.recs[vv[u]];
-                  }
-#pragma unroll
-                  for (int u = 0; u < RT_U; u++)
-                      Ls[u] = !(vv[u] >> 31) ? a.lines[rrs[u].line & 0x7fffffffu] : make_float4(0, 0, 0, 0);
-#pragma unroll
-                  for (int u = 0; u < RT_U; u++) {
-                    if (vv[u] >> 31) continue;
-                    const uint32_t x = xx[u];
-                    const RunRec rr = rrs[u];
-                    const float4 L = Ls[u];
-                    Dda d;
-                    const V2 last = mk(L.x, L.y), point = mk(L.z, L.w);
-                    if (rr.line >> 31) {
-                        dda_init(d, last, point);
-                    } else {
-                        dda_consts(d, last, point);
-                        d.x = (int16_t)(rr.xy & 0xffffu);
-                        d.y = (int16_t)(rr.xy >> 16);
-                        d.row_t1 = rr.row_t1;
-                        d.col_t1 = rr.col_t1;
-                        d.t0 = 0.0f;
-                        dda_step(d);
-                    }
-                    const int16_t rtx = div8(d.x), rty = div8(d.y);
-                    const bool finite = line_finite(last, point);
-                    V2 p0 = dda_point(d, d.t0);
-                    bool ended = false;
-                    for (int k = 0; k < 16; k++) {
-                        float area, height;
-                        dda_increment_carried(d, p0, finite, area, height);
-                        const uint32_t px = x * (uint32_t)UWS + (uint32_t)(((d.y & 7) << 3) | (d.x & 7));
-                        atomicAdd(&s.acc_a[px], area);
-                        atomicAdd(&s.acc_h[px], height);
-                        if (dda_step(d) || div8(d.x) != rtx || div8(d.y) != rty) {
-                            ended = true;
-                            break;
-                        }
-                    }
-                    if (!ended) atomicOr(&a.cnt->error, (uint32_t)ERR_REPLAY);
-                  }
-                }
-                __syncthreads();
                 // The height carry: every (occupied tile, pixel row) folds its 8
                 // heights into an in-tile prefix and a total, 8 lanes then carry
                 // the totals through the window's tiles left to right, and the
