@@ -378,7 +378,7 @@ struct DdaArgs {
     const float4* lines;
     const uint32_t* line_path;
     int64_t n_lines;
-    uint64_t* line_cnt;        // count pass: (runs << 32) | entries; emit pass: exclusive offsets
+    uint64_t* line_cnt;        // count pass: (runs << 32) | entries per line, read by the emit pass
     const PathGrid* grid;
     uint32_t xbits;
     uint32_t* keys;
@@ -386,35 +386,55 @@ struct DdaArgs {
     RunRec* recs;
     Counters* cnt;
     uint64_t* blk_entries;     // count pass: the entries of each block's lines
+    uint64_t* blk_cnt;         // count pass: each block's (runs << 32) | entries; emit pass: exclusive offsets
 };
 
+// Line offsets: the count pass leaves each line's packed (runs, entries)
+// count and each block's sum; the blocks' sums are scanned, and the emit pass
+// adds a block-local scan of its lines' counts (no device-wide scan over the
+// lines).  A block's packed sum cannot carry between the halves: 256 lines of
+// at most WALK_LIMIT entries each.
 template <bool EMIT>
 __global__ __launch_bounds__(256) void k_dda(DdaArgs a) {
     int64_t l = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    __shared__ unsigned long long part[4];
     if (!EMIT) {
-        unsigned long long e = 0;
+        unsigned long long e = 0, c = 0;
         if (l < a.n_lines) {
             float4 L = a.lines[l];
             CountSink s;
             s.cnt = a.cnt;
             walk_line(mk(L.x, L.y), mk(L.z, L.w), s);
-            a.line_cnt[l] = ((uint64_t)s.runs << 32) | s.entries;
+            c = ((uint64_t)s.runs << 32) | s.entries;
+            a.line_cnt[l] = c;
             e = s.entries;
         }
         // the 64-bit total of the block's entries (k_sum_u64 adds the blocks):
-        // the packed per-line counts are scanned in 32-bit halves, which wrap
+        // the packed block offsets are scanned in 32-bit halves, which wrap
         // silently past 2^32
-        for (int o = 32; o > 0; o >>= 1) e += __shfl_xor(e, o);
-        __shared__ unsigned long long part[4];
-        if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = e;
+        for (int o = 32; o > 0; o >>= 1) {
+            e += __shfl_xor(e, o);
+            c += __shfl_xor(c, o);
+        }
+        __shared__ unsigned long long cpart[4];
+        if ((threadIdx.x & 63) == 0) {
+            part[threadIdx.x >> 6] = e;
+            cpart[threadIdx.x >> 6] = c;
+        }
         __syncthreads();
-        if (threadIdx.x == 0) a.blk_entries[blockIdx.x] = part[0] + part[1] + part[2] + part[3];
+        if (threadIdx.x == 0) {
+            a.blk_entries[blockIdx.x] = part[0] + part[1] + part[2] + part[3];
+            a.blk_cnt[blockIdx.x] = cpart[0] + cpart[1] + cpart[2] + cpart[3];
+        }
         return;
     }
+    const uint64_t c = l < a.n_lines ? a.line_cnt[l] : 0ull;
+    unsigned long long btotal;
+    const uint64_t off = a.blk_cnt[blockIdx.x] +
+                         block_excl_scan<256, unsigned long long>(c, OpAdd(), 0ull, part, &btotal);
     if (l >= a.n_lines) return;
     float4 L = a.lines[l];
     {
-        uint64_t off = a.line_cnt[l];
         EmitSink s;
         s.line = (uint32_t)l;
         s.entry = (uint32_t)off;
@@ -1433,13 +1453,15 @@ static int run_plain_pass(Context* ctx, const SegInput& in, OutputDesc& out, hip
     da.xbits = 0;
     da.keys = da.vals = nullptr;
     da.recs = nullptr;
-    HIPCHK(ctx->dda_blk.ensure((size_t)(grid_for(std::max<int64_t>(L, 1), 256)) * 8));
+    const int64_t nblk = (int64_t)grid_for(std::max<int64_t>(L, 1), 256);
+    HIPCHK(ctx->dda_blk.ensure((size_t)nblk * 16));
     da.blk_entries = ctx->dda_blk.as<uint64_t>();
+    da.blk_cnt = da.blk_entries + nblk;
     if (L > 0) {
         hipLaunchKernelGGL(k_dda<false>, dim3(grid_for(L, 256)), dim3(256), 0, s, da);
-        hipLaunchKernelGGL(k_sum_u64, dim3(std::min<uint32_t>(grid_for(grid_for(L, 256), 256), 256)), dim3(256), 0, s,
-                           da.blk_entries, (int64_t)grid_for(L, 256), &cnt->entries);
-        HIPCHK(launch_scan<uint64_t>(ctx, L, LoadU64{da.line_cnt}, StoreU64{da.line_cnt}, OpAdd(), 0ull, &tot[2], s));
+        hipLaunchKernelGGL(k_sum_u64, dim3(std::min<uint32_t>(grid_for(nblk, 256), 256)), dim3(256), 0, s,
+                           da.blk_entries, nblk, &cnt->entries);
+        HIPCHK(launch_scan<uint64_t>(ctx, nblk, LoadU64{da.blk_cnt}, StoreU64{da.blk_cnt}, OpAdd(), 0ull, &tot[2], s));
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(&ht->rows, &tot[1], 4, hipMemcpyDeviceToHost, s));
