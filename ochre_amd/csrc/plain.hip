@@ -875,6 +875,7 @@ __global__ __launch_bounds__(WAVE) void k_tile_rows(TileArgs a) {
 // tile and the per-pixel-row height carry across them.
 constexpr int UW = 64;   // tiles per window
 constexpr int UWS = 65;  // LDS words per tile's sums (one of padding: lanes in different tiles hit different banks)
+static_assert(UW <= WAVE && UW % 32 == 0, "a window is at most one wave of tiles, in 32-bit occupancy words");
 
 struct LoadNWin {
     const PathGrid* g;
@@ -1073,8 +1074,9 @@ struct RowTileArgs {
     Counters* cnt;
 };
 
-constexpr int RT_NW = 4;
+constexpr int RT_NW = 4;  // wave w carries pixel rows 2w and 2w + 1
 constexpr int RT_NT = RT_NW * WAVE;
+static_assert(RT_NW * 2 == TILE, "the carry takes two pixel rows per wave");
 constexpr int RT_U = 4;  // entries per thread whose loads are issued together
 
 template <bool EMIT>
@@ -1147,7 +1149,7 @@ __global__ __launch_bounds__(RT_NT) void k_row_tiles(RowTileArgs a) {
         bool have_prev = false;
         int32_t prev_x = 0, prev_w = 0;
         uint32_t nt = 0, ns = 0;
-        float carry = 0.0f;  // wave 1, lanes 0-7: the height carry of pixel row `lane`
+        float carry0 = 0.0f, carry1 = 0.0f;  // wave w: the height carries of pixel rows 2w and 2w + 1
         for (uint32_t wi = 0; wi < nwin; wi++) {
             const uint32_t eb = a.bbeg[b0 + wi], ee = a.bbeg[b0 + wi + 1];
             if (ee == eb) continue;  // no run and no crossing in this window
@@ -1219,8 +1221,8 @@ __global__ __launch_bounds__(RT_NT) void k_row_tiles(RowTileArgs a) {
             if (wid == 0) {
                 // lane = tile x0 + lane: ranks, the winding after each tile,
                 // and the spans between consecutive tiles (rasterizer.rs:252-264)
-                const bool occ = (s.occ[lane >> 5] >> (lane & 31)) & 1u;
-                const int32_t incl = wave_incl_scan(s.tsign[lane], OpAdd());
+                const bool occ = lane < UW && ((s.occ[lane >> 5] >> (lane & 31)) & 1u);
+                const int32_t incl = wave_incl_scan(lane < UW ? s.tsign[lane] : 0, OpAdd());
                 const int32_t w_at = winding + incl;
                 const uint64_t om = __ballot(occ);
                 const uint64_t below = om & lt;
@@ -1260,29 +1262,31 @@ __global__ __launch_bounds__(RT_NT) void k_row_tiles(RowTileArgs a) {
             if constexpr (EMIT) {
                 __syncthreads();
                 const uint32_t nocc = s.nocc, nt_w = s.nt;
-                // The height carry: every (occupied tile, pixel row) folds its 8
-                // heights into an in-tile prefix and a total, 8 lanes then carry
-                // the totals through the window's tiles left to right, and the
+                // The height carry: wave w takes pixel rows 2w and 2w + 1, lane k
+                // the window's k-th occupied tile, which folds its row's 8
+                // heights into an in-tile prefix and a total; a wave scan over
+                // the tiles' totals gives each tile its incoming carry, and the
                 // quantise adds the two.  (Regrouping the row's f32 sums this way
                 // moves a pixel's carry by a few ulps: within the 1-LSB contract.)
-                for (uint32_t item = (uint32_t)tid; item < nocc * 8u; item += RT_NT) {
-                    const int base = (int)s.tlist[item >> 3] * UWS + (int)(item & 7u) * 8;
-                    float acc = 0.0f;
 #pragma unroll
-                    for (int k = 0; k < 8; k++) {
-                        const float h = s.acc_h[base + k];
-                        s.acc_h[base + k] = acc;
-                        acc += h;
+                for (int j = 0; j < 2; j++) {
+                    const int prow = 2 * wid + j;
+                    float tot = 0.0f;
+                    if ((uint32_t)lane < nocc) {
+                        const int base = (int)s.tlist[lane] * UWS + prow * 8;
+#pragma unroll
+                        for (int k = 0; k < 8; k++) {
+                            const float h = s.acc_h[base + k];
+                            s.acc_h[base + k] = tot;
+                            tot += h;
+                        }
                     }
-                    s.rtot[item] = acc;
-                }
-                __syncthreads();
-                if (wid == 1 && lane < 8) {
-                    for (uint32_t rk = 0; rk < nocc; rk++) {
-                        const float t = s.rtot[rk * 8 + lane];
-                        s.rtot[rk * 8 + lane] = carry;
-                        carry += t;
-                    }
+                    const float incl = wave_incl_scan(tot, OpAdd());
+                    const float up = __shfl_up(incl, 1, WAVE);
+                    const float excl = lane ? up : 0.0f;
+                    float& carry = j ? carry1 : carry0;
+                    if ((uint32_t)lane < nocc) s.rtot[lane * 8 + prow] = carry + excl;
+                    carry += __shfl(incl, WAVE - 1, WAVE);
                 }
                 __syncthreads();
                 for (uint32_t item = (uint32_t)tid; item < nocc * 16u; item += RT_NT) {
