@@ -873,8 +873,8 @@ __global__ __launch_bounds__(WAVE) void k_tile_rows(TileArgs a) {
 // the tile and span sets, which are integer, are exact.  One workgroup per tile
 // row walks the row's windows left to right, carrying the winding, the last
 // tile and the per-pixel-row height carry across them.
-constexpr int UW = 64;   // tiles per window
-constexpr int UWS = 65;  // LDS words per tile's sums (one of padding: lanes in different tiles hit different banks)
+constexpr int UW = 32;   // tiles per window
+constexpr int UWS = 65;  // accumulator slots per tile's sums (one of padding)
 static_assert(UW <= WAVE && UW % 32 == 0, "a window is at most one wave of tiles, in 32-bit occupancy words");
 
 struct LoadNWin {
@@ -1079,10 +1079,30 @@ constexpr int RT_NT = RT_NW * WAVE;
 static_assert(RT_NW * 2 == TILE, "the carry takes two pixel rows per wave");
 constexpr int RT_U = 4;  // entries per thread whose loads are issued together
 
+// The window's per-pixel sums are 64-bit fixed point (units of 2^-39): LDS
+// integer atomics cost a fraction of float ones, and the sums of the
+// increments (|area|, |height| <= 1 + a few ulps, each truncated at 2^-39) are
+// then exact, so that the decoded f32 sum is the correctly rounded sum of the
+// increments, independent of arrival order.  A window of a full-range path
+// (coordinates that may be huge or non-finite) or with 2^22 entries or more
+// keeps f32 atomics in the slots' low words instead.
+constexpr double RT_FX_UNIT = 1.8189894035458565e-12;  // 2^-39
+constexpr uint32_t RT_FX_MAX_ENTRIES = 1u << 22;      // 2^22 increments of |v| < 2 per pixel stay below 2^63
+
+__device__ inline long long rt_fx(float v) {
+    // v * 2^39 truncated toward zero, in two exact f32 steps: the integer
+    // part of v * 2^8, and the fraction scaled by 2^31
+    const float sv = v * 256.0f;
+    const float hi = truncf(sv);
+    const float lo = (sv - hi) * 2147483648.0f;
+    return (long long)(int)hi * 2147483648ll + (long long)(int)lo;
+}
+__device__ inline float rt_unfx(long long q) { return (float)((double)q * RT_FX_UNIT); }
+
 template <bool EMIT>
 struct RowLds {
-    float acc_a[EMIT ? UW * UWS : 4];  // tile t, pixel p at t * UWS + p
-    float acc_h[EMIT ? UW * UWS : 4];
+    long long acc_a[EMIT ? UW * UWS : 2];  // tile t, pixel p at t * UWS + p
+    long long acc_h[EMIT ? UW * UWS : 2];
     int32_t tsign[UW];
     uint32_t occ[UW / 32];
     uint8_t tlist[UW];  // the window's occupied tiles in x order
@@ -1093,7 +1113,8 @@ struct RowLds {
 // Replays a run (x: its tile within the window) into the window's LDS sums
 // with float atomics (arrival order).
 template <bool EMIT>
-__device__ inline void rt_replay_run(RowLds<EMIT>& s, uint32_t x, const RunRec& rr, const float4& L, Counters* cnt) {
+__device__ inline void rt_replay_run(RowLds<EMIT>& s, uint32_t x, const RunRec& rr, const float4& L, bool fx,
+                                     Counters* cnt) {
     Dda d;
     const V2 last = mk(L.x, L.y), point = mk(L.z, L.w);
     if (rr.line >> 31) {
@@ -1115,8 +1136,13 @@ __device__ inline void rt_replay_run(RowLds<EMIT>& s, uint32_t x, const RunRec& 
         float area, height;
         dda_increment_carried(d, p0, finite, area, height);
         const uint32_t px = x * (uint32_t)UWS + (uint32_t)(((d.y & 7) << 3) | (d.x & 7));
-        atomicAdd(&s.acc_a[px], area);
-        atomicAdd(&s.acc_h[px], height);
+        if (fx) {
+            atomicAdd(reinterpret_cast<unsigned long long*>(&s.acc_a[px]), (unsigned long long)rt_fx(area));
+            atomicAdd(reinterpret_cast<unsigned long long*>(&s.acc_h[px]), (unsigned long long)rt_fx(height));
+        } else {
+            atomicAdd(reinterpret_cast<float*>(&s.acc_a[px]), area);
+            atomicAdd(reinterpret_cast<float*>(&s.acc_h[px]), height);
+        }
         if (dda_step(d) || div8(d.x) != rtx || div8(d.y) != rty) {
             ended = true;
             break;
@@ -1154,14 +1180,17 @@ __global__ __launch_bounds__(RT_NT) void k_row_tiles(RowTileArgs a) {
             const uint32_t eb = a.bbeg[b0 + wi], ee = a.bbeg[b0 + wi + 1];
             if (ee == eb) continue;  // no run and no crossing in this window
             const int32_t x0 = (int32_t)(wi * UW);
+            const bool fx = G.w < 8192 && ee - eb < RT_FX_MAX_ENTRIES;  // fixed-point sums (see RowLds)
             if (tid < UW) s.tsign[tid] = 0;
             if (tid < UW / 32) s.occ[tid] = 0;
             if constexpr (EMIT) {
                 // all the window's sums start at zero (the runs add into them
                 // before the occupied tiles are known)
+                constexpr int n16 = (int)sizeof(s.acc_a) / 16;  // the sums are cleared 16 bytes at a time
+                static_assert(sizeof(s.acc_a) % 16 == 0, "16-byte clears");
                 float4* za = reinterpret_cast<float4*>(s.acc_a);
                 float4* zh = reinterpret_cast<float4*>(s.acc_h);
-                for (int k = tid; k < UW * UWS / 4; k += RT_NT) {
+                for (int k = tid; k < n16; k += RT_NT) {
                     za[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                     zh[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                 }
@@ -1200,7 +1229,7 @@ __global__ __launch_bounds__(RT_NT) void k_row_tiles(RowTileArgs a) {
                             if (sg) atomicAdd(&s.tsign[x], sg);
                         } else {
                             atomicOr(&s.occ[x >> 5], 1u << (x & 31));
-                            rt_replay_run(s, x, rrs[u], Ls[u], a.cnt);
+                            rt_replay_run(s, x, rrs[u], Ls[u], fx, a.cnt);
                         }
                     }
                 } else {
@@ -1274,10 +1303,11 @@ __global__ __launch_bounds__(RT_NT) void k_row_tiles(RowTileArgs a) {
                     float tot = 0.0f;
                     if ((uint32_t)lane < nocc) {
                         const int base = (int)s.tlist[lane] * UWS + prow * 8;
+                        float* fh = reinterpret_cast<float*>(s.acc_h);  // the slots' low words
 #pragma unroll
                         for (int k = 0; k < 8; k++) {
-                            const float h = s.acc_h[base + k];
-                            s.acc_h[base + k] = tot;
+                            const float h = fx ? rt_unfx(s.acc_h[base + k]) : fh[2 * (base + k)];
+                            fh[2 * (base + k)] = tot;
                             tot += h;
                         }
                     }
@@ -1296,9 +1326,12 @@ __global__ __launch_bounds__(RT_NT) void k_row_tiles(RowTileArgs a) {
                     const int base = t * UWS + (int)quad * 4;
                     uint32_t word = 0;
                     const float cin = s.rtot[rk * 8 + (quad >> 1)];  // pixel row quad / 2
+                    const float* fh = reinterpret_cast<const float*>(s.acc_h);
+                    const float* fa = reinterpret_cast<const float*>(s.acc_a);
 #pragma unroll
                     for (int k = 0; k < 4; k++) {
-                        const float v = (cin + s.acc_h[base + k]) + s.acc_a[base + k];
+                        const float av = fx ? rt_unfx(s.acc_a[base + k]) : fa[2 * (base + k)];
+                        const float v = (cin + fh[2 * (base + k)]) + av;
                         word |= (uint32_t)fminf(fabsf(v) * 256.0f, 255.0f) << (8 * k);
                     }
                     const uint64_t ti = (uint64_t)row_tiles + nt_w + rk;
