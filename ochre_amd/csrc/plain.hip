@@ -1074,9 +1074,10 @@ struct RowTileArgs {
     Counters* cnt;
 };
 
-constexpr int RT_NW = 4;  // wave w carries pixel rows 2w and 2w + 1
+constexpr int RT_NW = 4;
 constexpr int RT_NT = RT_NW * WAVE;
-static_assert(RT_NW * 2 == TILE, "the carry takes two pixel rows per wave");
+constexpr int RT_ROWS = TILE / RT_NW;  // pixel rows whose carry each wave takes
+static_assert(RT_ROWS * RT_NW == TILE && RT_ROWS <= 2, "the carry takes one or two pixel rows per wave");
 constexpr int RT_U = 4;  // entries per thread whose loads are issued together
 
 // The window's per-pixel sums are 64-bit fixed point (units of 2^-39): LDS
@@ -1086,18 +1087,7 @@ constexpr int RT_U = 4;  // entries per thread whose loads are issued together
 // increments, independent of arrival order.  A window of a full-range path
 // (coordinates that may be huge or non-finite) or with 2^22 entries or more
 // keeps f32 atomics in the slots' low words instead.
-constexpr double RT_FX_UNIT = 1.8189894035458565e-12;  // 2^-39
-constexpr uint32_t RT_FX_MAX_ENTRIES = 1u << 22;      // 2^22 increments of |v| < 2 per pixel stay below 2^63
-
-__device__ inline long long rt_fx(float v) {
-    // v * 2^39 truncated toward zero, in two exact f32 steps: the integer
-    // part of v * 2^8, and the fraction scaled by 2^31
-    const float sv = v * 256.0f;
-    const float hi = truncf(sv);
-    const float lo = (sv - hi) * 2147483648.0f;
-    return (long long)(int)hi * 2147483648ll + (long long)(int)lo;
-}
-__device__ inline float rt_unfx(long long q) { return (float)((double)q * RT_FX_UNIT); }
+constexpr uint32_t RT_FX_MAX_ENTRIES = 1u << 22;  // 2^22 increments of |v| < 2 per pixel stay below 2^63
 
 template <bool EMIT>
 struct RowLds {
@@ -1137,8 +1127,8 @@ __device__ inline void rt_replay_run(RowLds<EMIT>& s, uint32_t x, const RunRec& 
         dda_increment_carried(d, p0, finite, area, height);
         const uint32_t px = x * (uint32_t)UWS + (uint32_t)(((d.y & 7) << 3) | (d.x & 7));
         if (fx) {
-            atomicAdd(reinterpret_cast<unsigned long long*>(&s.acc_a[px]), (unsigned long long)rt_fx(area));
-            atomicAdd(reinterpret_cast<unsigned long long*>(&s.acc_h[px]), (unsigned long long)rt_fx(height));
+            atomicAdd(reinterpret_cast<unsigned long long*>(&s.acc_a[px]), (unsigned long long)fx_of(area));
+            atomicAdd(reinterpret_cast<unsigned long long*>(&s.acc_h[px]), (unsigned long long)fx_of(height));
         } else {
             atomicAdd(reinterpret_cast<float*>(&s.acc_a[px]), area);
             atomicAdd(reinterpret_cast<float*>(&s.acc_h[px]), height);
@@ -1175,7 +1165,7 @@ __global__ __launch_bounds__(RT_NT) void k_row_tiles(RowTileArgs a) {
         bool have_prev = false;
         int32_t prev_x = 0, prev_w = 0;
         uint32_t nt = 0, ns = 0;
-        float carry0 = 0.0f, carry1 = 0.0f;  // wave w: the height carries of pixel rows 2w and 2w + 1
+        float carry0 = 0.0f, carry1 = 0.0f;  // wave w: the height carries of pixel rows RT_ROWS * w + j
         for (uint32_t wi = 0; wi < nwin; wi++) {
             const uint32_t eb = a.bbeg[b0 + wi], ee = a.bbeg[b0 + wi + 1];
             if (ee == eb) continue;  // no run and no crossing in this window
@@ -1291,22 +1281,22 @@ __global__ __launch_bounds__(RT_NT) void k_row_tiles(RowTileArgs a) {
             if constexpr (EMIT) {
                 __syncthreads();
                 const uint32_t nocc = s.nocc, nt_w = s.nt;
-                // The height carry: wave w takes pixel rows 2w and 2w + 1, lane k
-                // the window's k-th occupied tile, which folds its row's 8
+                // The height carry: wave w takes RT_ROWS pixel rows from RT_ROWS * w,
+                // lane k the window's k-th occupied tile, which folds its row's 8
                 // heights into an in-tile prefix and a total; a wave scan over
                 // the tiles' totals gives each tile its incoming carry, and the
                 // quantise adds the two.  (Regrouping the row's f32 sums this way
                 // moves a pixel's carry by a few ulps: within the 1-LSB contract.)
 #pragma unroll
-                for (int j = 0; j < 2; j++) {
-                    const int prow = 2 * wid + j;
+                for (int j = 0; j < RT_ROWS; j++) {
+                    const int prow = RT_ROWS * wid + j;
                     float tot = 0.0f;
                     if ((uint32_t)lane < nocc) {
                         const int base = (int)s.tlist[lane] * UWS + prow * 8;
                         float* fh = reinterpret_cast<float*>(s.acc_h);  // the slots' low words
 #pragma unroll
                         for (int k = 0; k < 8; k++) {
-                            const float h = fx ? rt_unfx(s.acc_h[base + k]) : fh[2 * (base + k)];
+                            const float h = fx ? fx_to_f(s.acc_h[base + k]) : fh[2 * (base + k)];
                             fh[2 * (base + k)] = tot;
                             tot += h;
                         }
@@ -1330,7 +1320,7 @@ __global__ __launch_bounds__(RT_NT) void k_row_tiles(RowTileArgs a) {
                     const float* fa = reinterpret_cast<const float*>(s.acc_a);
 #pragma unroll
                     for (int k = 0; k < 4; k++) {
-                        const float av = fx ? rt_unfx(s.acc_a[base + k]) : fa[2 * (base + k)];
+                        const float av = fx ? fx_to_f(s.acc_a[base + k]) : fa[2 * (base + k)];
                         const float v = (cin + fh[2 * (base + k)]) + av;
                         word |= (uint32_t)fminf(fabsf(v) * 256.0f, 255.0f) << (8 * k);
                     }
