@@ -1078,7 +1078,7 @@ constexpr int RT_NW = 4;
 constexpr int RT_NT = RT_NW * WAVE;
 constexpr int RT_ROWS = TILE / RT_NW;  // pixel rows whose carry each wave takes
 static_assert(RT_ROWS * RT_NW == TILE && RT_ROWS <= 2, "the carry takes one or two pixel rows per wave");
-constexpr int RT_U = 4;  // entries per thread whose loads are issued together
+constexpr int RT_U = 2;  // entries per thread whose loads are issued together
 
 // The window's per-pixel sums are 64-bit fixed point (units of 2^-39): LDS
 // integer atomics cost a fraction of float ones, and the sums of the
@@ -1141,6 +1141,57 @@ __device__ inline void rt_replay_run(RowLds<EMIT>& s, uint32_t x, const RunRec& 
     if (!ended) atomicOr(&cnt->error, (uint32_t)ERR_REPLAY);
 }
 
+// RT_U entries per thread of a window (entries e0 + u * RT_NT), with their
+// run records and lines, loaded level by level (each level's loads in flight
+// together).  The emit pass loads the next window's first batch while the
+// current window's ranks, carries and quantisation run.
+struct RtBatch {
+    uint32_t kk[RT_U], vv[RT_U];
+    RunRec rr[RT_U];
+    float4 L[RT_U];
+};
+__device__ inline void rt_load_keys(RtBatch& b, const RowTileArgs& a, uint32_t e0, uint32_t ee) {
+#pragma unroll
+    for (int u = 0; u < RT_U; u++) {
+        const uint32_t e = e0 + (uint32_t)(u * RT_NT);
+        b.kk[u] = e < ee ? a.keys[e] : 0u;
+        b.vv[u] = e < ee ? a.vals[e] : 0x80000000u;
+    }
+}
+__device__ inline void rt_load_recs(RtBatch& b, const RowTileArgs& a) {
+#pragma unroll
+    for (int u = 0; u < RT_U; u++) {
+        b.rr[u].line = 0;
+        b.rr[u].xy = 0;
+        b.rr[u].row_t1 = b.rr[u].col_t1 = 0.0f;
+        if (!(b.vv[u] >> 31)) b.rr[u] = a.recs[b.vv[u]];
+    }
+}
+__device__ inline void rt_load_lines(RtBatch& b, const RowTileArgs& a) {
+#pragma unroll
+    for (int u = 0; u < RT_U; u++)
+        b.L[u] = !(b.vv[u] >> 31) ? a.lines[b.rr[u].line & 0x7fffffffu] : make_float4(0, 0, 0, 0);
+}
+
+// One batch of a window's entries: occupancy and crossing sums from the keys,
+// and (emit pass) the runs replayed.
+template <bool EMIT>
+__device__ inline void rt_entries(RowLds<EMIT>& s, const RtBatch& b, uint32_t e0, uint32_t ee, uint32_t xmask,
+                                  int32_t x0, bool fx, Counters* cnt) {
+#pragma unroll
+    for (int u = 0; u < RT_U; u++) {
+        if (e0 + (uint32_t)(u * RT_NT) >= ee) break;
+        const uint32_t x = (b.kk[u] & xmask) - (uint32_t)x0;
+        if (b.vv[u] >> 31) {
+            const int sg = (int)(int8_t)(b.vv[u] & 0xffu);
+            if (sg) atomicAdd(&s.tsign[x], sg);
+        } else {
+            atomicOr(&s.occ[x >> 5], 1u << (x & 31));
+            if constexpr (EMIT) rt_replay_run(s, x, b.rr[u], b.L[u], fx, cnt);
+        }
+    }
+}
+
 template <bool EMIT>
 __global__ __launch_bounds__(RT_NT) void k_row_tiles(RowTileArgs a) {
     __shared__ __align__(16) RowLds<EMIT> s;
@@ -1166,9 +1217,24 @@ __global__ __launch_bounds__(RT_NT) void k_row_tiles(RowTileArgs a) {
         int32_t prev_x = 0, prev_w = 0;
         uint32_t nt = 0, ns = 0;
         float carry0 = 0.0f, carry1 = 0.0f;  // wave w: the height carries of pixel rows RT_ROWS * w + j
-        for (uint32_t wi = 0; wi < nwin; wi++) {
-            const uint32_t eb = a.bbeg[b0 + wi], ee = a.bbeg[b0 + wi + 1];
-            if (ee == eb) continue;  // no run and no crossing in this window
+        // the first window holding entries (a window with none has no run
+        // and no crossing)
+        uint32_t wi = 0, eb = a.bbeg[b0], ee = a.bbeg[b0 + 1];
+        while (ee == eb && ++wi < nwin) {
+            eb = ee;
+            ee = a.bbeg[b0 + wi + 1];
+        }
+        RtBatch nb;       // the first batch of window wi, when pf
+        bool pf = false;
+        while (wi < nwin) {
+            // the next window holding entries
+            uint32_t wn = wi + 1, nbeg = ee, nend = ee;
+            while (wn < nwin) {
+                nend = a.bbeg[b0 + wn + 1];
+                if (nend != nbeg) break;
+                wn++;
+            }
+            const bool has_next = EMIT && wn < nwin;
             const int32_t x0 = (int32_t)(wi * UW);
             const bool fx = G.w < 8192 && ee - eb < RT_FX_MAX_ENTRIES;  // fixed-point sums (see RowLds)
             if (tid < UW) s.tsign[tid] = 0;
@@ -1186,57 +1252,23 @@ __global__ __launch_bounds__(RT_NT) void k_row_tiles(RowTileArgs a) {
                 }
             }
             __syncthreads();
-            // one pass over the window's entries, RT_U per thread at a time
-            // (each level of their loads in flight together): occupancy and
-            // crossing sums from the keys, and (emit pass) the runs replayed
-            for (uint32_t e0 = eb + (uint32_t)tid; e0 < ee; e0 += RT_U * RT_NT) {
-                uint32_t kk[RT_U], vv[RT_U];
-#pragma unroll
-                for (int u = 0; u < RT_U; u++) {
-                    const uint32_t e = e0 + (uint32_t)(u * RT_NT);
-                    kk[u] = e < ee ? a.keys[e] : 0u;
-                    vv[u] = e < ee ? a.vals[e] : 0x80000000u;
-                }
+            // one pass over the window's entries, a batch at a time
+            uint32_t e0 = eb + (uint32_t)tid;
+            if (pf) {
+                rt_entries(s, nb, e0, ee, xmask, x0, fx, a.cnt);
+                e0 += RT_U * RT_NT;
+            }
+            for (; e0 < ee; e0 += RT_U * RT_NT) {
+                RtBatch b;
+                rt_load_keys(b, a, e0, ee);
                 if constexpr (EMIT) {
-                    RunRec rrs[RT_U];
-#pragma unroll
-                    for (int u = 0; u < RT_U; u++) {
-                        rrs[u].line = 0;
-                        rrs[u].xy = 0;
-                        rrs[u].row_t1 = rrs[u].col_t1 = 0.0f;
-                        if (!(vv[u] >> 31)) rrs[u] = a.recs[vv[u]];
-                    }
-                    float4 Ls[RT_U];
-#pragma unroll
-                    for (int u = 0; u < RT_U; u++)
-                        Ls[u] = !(vv[u] >> 31) ? a.lines[rrs[u].line & 0x7fffffffu] : make_float4(0, 0, 0, 0);
-#pragma unroll
-                    for (int u = 0; u < RT_U; u++) {
-                        if (e0 + (uint32_t)(u * RT_NT) >= ee) break;
-                        const uint32_t x = (kk[u] & xmask) - (uint32_t)x0;
-                        if (vv[u] >> 31) {
-                            const int sg = (int)(int8_t)(vv[u] & 0xffu);
-                            if (sg) atomicAdd(&s.tsign[x], sg);
-                        } else {
-                            atomicOr(&s.occ[x >> 5], 1u << (x & 31));
-                            rt_replay_run(s, x, rrs[u], Ls[u], fx, a.cnt);
-                        }
-                    }
-                } else {
-#pragma unroll
-                    for (int u = 0; u < RT_U; u++) {
-                        if (e0 + (uint32_t)(u * RT_NT) >= ee) break;
-                        const uint32_t x = (kk[u] & xmask) - (uint32_t)x0;
-                        if (vv[u] >> 31) {
-                            const int sg = (int)(int8_t)(vv[u] & 0xffu);
-                            if (sg) atomicAdd(&s.tsign[x], sg);
-                        } else {
-                            atomicOr(&s.occ[x >> 5], 1u << (x & 31));
-                        }
-                    }
+                    rt_load_recs(b, a);
+                    rt_load_lines(b, a);
                 }
+                rt_entries(s, b, e0, ee, xmask, x0, fx, a.cnt);
             }
             __syncthreads();
+            if (has_next) rt_load_keys(nb, a, nbeg + (uint32_t)tid, nend);
             if (wid == 0) {
                 // lane = tile x0 + lane: ranks, the winding after each tile,
                 // and the spans between consecutive tiles (rasterizer.rs:252-264)
@@ -1280,6 +1312,7 @@ __global__ __launch_bounds__(RT_NT) void k_row_tiles(RowTileArgs a) {
             }
             if constexpr (EMIT) {
                 __syncthreads();
+                if (has_next) rt_load_recs(nb, a);
                 const uint32_t nocc = s.nocc, nt_w = s.nt;
                 // The height carry: wave w takes RT_ROWS pixel rows from RT_ROWS * w,
                 // lane k the window's k-th occupied tile, which folds its row's 8
@@ -1309,6 +1342,7 @@ __global__ __launch_bounds__(RT_NT) void k_row_tiles(RowTileArgs a) {
                     carry += __shfl(incl, WAVE - 1, WAVE);
                 }
                 __syncthreads();
+                if (has_next) rt_load_lines(nb, a);
                 for (uint32_t item = (uint32_t)tid; item < nocc * 16u; item += RT_NT) {
                     // item -> (the occupied tile of rank item / 16, 4 pixels)
                     const uint32_t rk = item >> 4, quad = item & 15u;
@@ -1333,6 +1367,10 @@ __global__ __launch_bounds__(RT_NT) void k_row_tiles(RowTileArgs a) {
                 }
             }
             __syncthreads();
+            pf = has_next;
+            wi = wn;
+            eb = nbeg;
+            ee = nend;
         }
         if (!EMIT && tid == 0) a.row_cnt[row] = ((uint64_t)nt << 32) | ns;
     }
