@@ -746,6 +746,7 @@ int ochre_rasterizer_finish(ochre_rasterizer* r, const ochre_tile_builder* build
         si.n_points = (int64_t)r->points.size() / 2;
         si.n_weights = (int64_t)r->weights.size();
         OutputDesc od;
+        const bool one_fill = nseg == 1 && !r->seg_raw[0] && r->s_width.empty();
         int64_t tcap = (int64_t)(dp->tile_alpha.bytes / 64), scap = (int64_t)(dp->span_w.bytes / 2);
         for (int attempt = 0; attempt < 2; attempt++) {
             hipError_t e;
@@ -768,7 +769,22 @@ int ochre_rasterizer_finish(ochre_rasterizer* r, const ochre_tile_builder* build
             od.path_tile_count = dp->ptc.as<int32_t>();
             od.path_span_start = dp->pss.as<int64_t>();
             od.path_span_count = dp->psc.as<int32_t>();
-            st = run_plain(&ctx->c, si, od, s);
+            if (one_fill) {
+                // one fill (the reference's demo makes a Rasterizer per fill):
+                // the ordered batched kernel, bit-exact like the plain pipeline,
+                // in one launch and one host synchronisation; a path over its
+                // limits goes to the plain pipeline within the call
+                SegInput s1 = si;
+                s1.seg_path = nullptr;
+                s1.path_seg_off = nullptr;
+                const bool dense = ctx->c.dense;
+                ctx->c.dense = false;
+                st = run_batched(&ctx->c, s1, od, s, ctx->c.batched_waves_per_cu, ctx->c.batched_gran,
+                                 ctx->c.batched_tc, false);
+                ctx->c.dense = dense;
+            } else {
+                st = run_plain(&ctx->c, si, od, s);
+            }
             if (st != OCHRE_ERR_CAPACITY) break;
             tcap = od.n_tiles;
             scap = od.n_spans;
