@@ -48,21 +48,38 @@ __global__ __launch_bounds__(RS_NT) void k_rs_count(const uint32_t* __restrict__
 
 // Scatter.  offsets[digit * nblocks + block] = exclusive global start.
 // vals_in == nullptr means the identity permutation (value = index).
-__global__ __launch_bounds__(RS_NT) void k_rs_scatter(const uint32_t* __restrict__ keys_in,
+// The block's keys are first placed in LDS in digit order (block-local stable
+// positions: digit start + the wave's offset in the digit + the rank within
+// the wave), then written out in that order, so that consecutive threads
+// store consecutive addresses of one digit's run; the values follow through
+// the same positions.
+static_assert(RS_NT == 256, "the scatter takes one digit per thread");
+
+struct RsScatterLds {
+    uint32_t cnt[RS_WAVES][256];  // per wave and digit: count, then the wave's start within the digit
+    uint32_t dstart[256];         // the digit's first position in the block
+    uint32_t goff[256];           // the digit's global start for this block
+    uint32_t buf[RS_TILE];        // keys, then values, in digit order
+    uint8_t dig[RS_TILE];         // the digit at each position
+};
+
+__global__ __launch_bounds__(RS_NT, 4) void k_rs_scatter(const uint32_t* __restrict__ keys_in,
                                                       const uint32_t* __restrict__ vals_in, int64_t n, int shift,
                                                       const uint32_t* __restrict__ offsets, uint32_t nblocks,
                                                       uint32_t* __restrict__ keys_out,
                                                       uint32_t* __restrict__ vals_out) {
-    __shared__ uint32_t cnt[RS_WAVES][256];
-    __shared__ uint32_t wave_off[RS_WAVES][256];
+    __shared__ RsScatterLds s;
     const int tid = threadIdx.x;
     const int wid = tid / WAVE;
     const int lane = lane_id();
-    for (int i = tid; i < RS_WAVES * 256; i += RS_NT) (&cnt[0][0])[i] = 0;
+    for (int i = tid; i < RS_WAVES * 256; i += RS_NT) (&s.cnt[0][0])[i] = 0;
+    for (int d = tid; d < 256; d += RS_NT) s.goff[d] = offsets[(int64_t)d * nblocks + blockIdx.x];
     __syncthreads();
     const uint64_t lt = lanemask_lt();
     // Items of wave w: base + w*1024 + j*64 + lane, so (j, lane) order is index order.
-    const int64_t wbase = (int64_t)blockIdx.x * RS_TILE + (int64_t)wid * RS_WAVE_ITEMS;
+    const int64_t bbase = (int64_t)blockIdx.x * RS_TILE;
+    const int64_t wbase = bbase + (int64_t)wid * RS_WAVE_ITEMS;
+    const int nb = (int)(n - bbase < (int64_t)RS_TILE ? n - bbase : (int64_t)RS_TILE);  // items in this block
     uint32_t key[RS_ITEMS];
     uint32_t rank[RS_ITEMS];
 #pragma unroll
@@ -78,30 +95,62 @@ __global__ __launch_bounds__(RS_NT) void k_rs_scatter(const uint32_t* __restrict
             uint64_t m = __ballot((d >> b) & 1u);
             match &= ((d >> b) & 1u) ? m : ~m;
         }
-        uint32_t prior = valid ? cnt[wid][d] : 0u;
+        uint32_t prior = valid ? s.cnt[wid][d] : 0u;
         rank[j] = prior + (uint32_t)__popcll(match & lt);
         // The lowest lane of each match group advances the wave-private counter.
-        if (valid && (match & lt) == 0) cnt[wid][d] = prior + (uint32_t)__popcll(match);
+        if (valid && (match & lt) == 0) s.cnt[wid][d] = prior + (uint32_t)__popcll(match);
     }
     __syncthreads();
-    for (int d = tid; d < 256; d += RS_NT) {
-        uint32_t s = 0;
+    // per digit: the waves' starts within the digit, and the digit's total
+    uint32_t tot = 0;
+    {
+        const int d = tid;  // RS_NT == 256: one thread per digit
 #pragma unroll
         for (int w = 0; w < RS_WAVES; w++) {
-            wave_off[w][d] = s;
-            s += cnt[w][d];
+            const uint32_t c = s.cnt[w][d];
+            s.cnt[w][d] = tot;
+            tot += c;
         }
+    }
+    // the digits' starts in the block: a block scan of the totals
+    {
+        const uint32_t incl = wave_incl_scan(tot, OpAdd());
+        if (lane == WAVE - 1) s.dstart[wid] = incl;  // wave totals (dstart reused below)
+        __syncthreads();
+        uint32_t before = 0;
+#pragma unroll
+        for (int w = 0; w < RS_WAVES; w++)
+            if (w < wid) before += s.dstart[w];
+        __syncthreads();
+        s.dstart[tid] = before + incl - tot;
+    }
+    __syncthreads();
+    // rank[j] becomes the item's position in the block
+#pragma unroll
+    for (int j = 0; j < RS_ITEMS; j++) {
+        int64_t i = wbase + (int64_t)j * WAVE + lane;
+        if (i < n) {
+            const uint32_t d = (key[j] >> shift) & 255u;
+            rank[j] += s.dstart[d] + s.cnt[wid][d];
+            s.buf[rank[j]] = key[j];
+            s.dig[rank[j]] = (uint8_t)d;
+        }
+    }
+    __syncthreads();
+    for (int i = tid; i < nb; i += RS_NT) {
+        const uint32_t d = s.dig[i];
+        keys_out[s.goff[d] + (uint32_t)i - s.dstart[d]] = s.buf[i];
     }
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < RS_ITEMS; j++) {
         int64_t i = wbase + (int64_t)j * WAVE + lane;
-        if (i < n) {
-            uint32_t d = (key[j] >> shift) & 255u;
-            uint32_t pos = offsets[(int64_t)d * nblocks + blockIdx.x] + wave_off[wid][d] + rank[j];
-            keys_out[pos] = key[j];
-            vals_out[pos] = vals_in ? vals_in[i] : (uint32_t)i;
-        }
+        if (i < n) s.buf[rank[j]] = vals_in ? vals_in[i] : (uint32_t)i;
+    }
+    __syncthreads();
+    for (int i = tid; i < nb; i += RS_NT) {
+        const uint32_t d = s.dig[i];
+        vals_out[s.goff[d] + (uint32_t)i - s.dstart[d]] = s.buf[i];
     }
 }
 
