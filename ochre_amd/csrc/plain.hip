@@ -476,7 +476,12 @@ struct LoadGroupStart {
     const uint32_t* keys;
     __device__ uint32_t operator()(int64_t i) const { return (i == 0 || keys[i] != keys[i - 1]) ? 1u : 0u; }
 };
+// Group tables from the sorted entries: each group's first entry and key,
+// whether it holds a run, and its crossing sum.  A thread's chunk of
+// consecutive entries is taken at once: one sign atomic and one run flag per
+// group the chunk touches, instead of one per entry.
 struct StoreGroup {
+    using chunk_tag = void;
     const uint32_t* keys;
     const uint32_t* vals;
     uint32_t* gbegin;
@@ -484,20 +489,39 @@ struct StoreGroup {
     uint32_t* ghasrun;
     int32_t* gsign;
     int64_t n;
-    __device__ void operator()(int64_t i, uint32_t excl, uint32_t flag) const {
-        uint32_t gid = excl + flag - 1;
-        if (flag) {
-            gbegin[gid] = (uint32_t)i;
-            gkey[gid] = keys[i];
+    __device__ void flush(uint32_t gid, int32_t sg, bool run) const {
+        if (sg) atomicAdd(&gsign[gid], sg);
+        if (run) ghasrun[gid] = 1u;
+    }
+    // items [base, base + SCAN_ITEMS) of flags v (1: a group starts), r = the
+    // groups started before base
+    __device__ void chunk(int64_t base, int64_t n_, uint32_t r, const uint32_t* v) const {
+        uint32_t gid = 0;
+        int32_t sg = 0;
+        bool run = false, open = false;
+        for (int j = 0; j < SCAN_ITEMS; j++) {
+            const int64_t i = base + j;
+            if (i >= n_) break;
+            if (v[j]) {
+                if (open) flush(gid, sg, run);
+                gid = r;
+                gbegin[gid] = (uint32_t)i;
+                gkey[gid] = keys[i];
+                sg = 0;
+                run = false;
+            } else if (!open) {
+                gid = r - 1;  // the group continues from the previous chunk
+            }
+            open = true;
+            r += v[j];
+            const uint32_t val = vals[i];
+            if (val >> 31)
+                sg += (int8_t)(val & 0xffu);
+            else
+                run = true;
+            if (i == n_ - 1) gbegin[gid + 1] = (uint32_t)n_;
         }
-        if (i == n - 1) gbegin[gid + 1] = (uint32_t)n;
-        uint32_t v = vals[i];
-        if (v >> 31) {
-            int s = (int8_t)(v & 0xffu);
-            if (s) atomicAdd(&gsign[gid], s);
-        } else {
-            ghasrun[gid] = 1u;
-        }
+        if (open) flush(gid, sg, run);
     }
 };
 

@@ -5,6 +5,8 @@
 #include <stdint.h>
 #include <string.h>
 
+#include <type_traits>
+
 namespace ochre {
 
 constexpr int WAVE = 64;
@@ -83,6 +85,28 @@ constexpr int SCAN_NT = 256;
 constexpr int SCAN_ITEMS = 16;
 constexpr int SCAN_TILE = SCAN_NT * SCAN_ITEMS;
 
+// A store functor with a `chunk_tag` type member takes a thread's whole run
+// of SCAN_ITEMS consecutive items at once (chunk(base, n, excl, v)), so that
+// it can aggregate across them; others take one item at a time.
+template <typename S, typename = void>
+struct IsChunkStore : std::false_type {};
+template <typename S>
+struct IsChunkStore<S, std::void_t<typename S::chunk_tag>> : std::true_type {};
+
+template <typename T, typename Op, typename Store>
+__device__ inline void tile_store(const Store& store, int64_t base, int64_t n, T r, const T* v, Op op) {
+    if constexpr (IsChunkStore<Store>::value) {
+        store.chunk(base, n, r, v);
+    } else {
+#pragma unroll
+        for (int j = 0; j < SCAN_ITEMS; j++) {
+            int64_t i = base + j;
+            if (i < n) store(i, r, v[j]);
+            r = op(r, v[j]);
+        }
+    }
+}
+
 template <typename T, typename Op, typename Load>
 __device__ inline T tile_load_reduce(Load load, int64_t base, int64_t n, Op op, T identity, T* v) {
     T acc = identity;
@@ -143,12 +167,7 @@ __global__ __launch_bounds__(SCAN_NT) void k_scan_down(Load load, Store store, i
     T acc = tile_load_reduce<T>(load, base, n, op, identity, v);
     T total;
     T r = op(partials ? partials[blockIdx.x] : identity, block_excl_scan<SCAN_NT>(acc, op, identity, s_red, &total));
-#pragma unroll
-    for (int j = 0; j < SCAN_ITEMS; j++) {
-        int64_t i = base + j;
-        if (i < n) store(i, r, v[j]);
-        r = op(r, v[j]);
-    }
+    tile_store(store, base, n, r, v, op);
 }
 
 // Single-tile scan that also writes the total.
@@ -161,12 +180,7 @@ __global__ __launch_bounds__(SCAN_NT) void k_scan_single(Load load, Store store,
     T acc = tile_load_reduce<T>(load, base, n, op, identity, v);
     T total;
     T r = block_excl_scan<SCAN_NT>(acc, op, identity, s_red, &total);
-#pragma unroll
-    for (int j = 0; j < SCAN_ITEMS; j++) {
-        int64_t i = base + j;
-        if (i < n) store(i, r, v[j]);
-        r = op(r, v[j]);
-    }
+    tile_store(store, base, n, r, v, op);
     if (threadIdx.x == 0 && total_out != nullptr) *total_out = total;
 }
 
