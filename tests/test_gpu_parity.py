@@ -188,6 +188,32 @@ def test_rasterizer_api_event_order(gpu_ctx):
                     assert a[3] == b[3]
 
 
+def test_rasterizer_api_single_large_fill(gpu_ctx):
+    """A single fill goes through the ordered batched kernel; a path over its
+    limits (here about 80 x 80 tiles) is handed to the plain pipeline inside the
+    same call, and the events still match the reference's exactly."""
+    from ochre_amd import EventRecorder, Rasterizer
+    b = W.star_blobs(n=1, seed=5, nmin=1500, nmax=1500, rmin=300.0, rmax=300.0)
+    v, p, w, xf = b.path(0)
+    r = Rasterizer()
+    r.fill_arrays(v, p, w, xf)
+    rec = EventRecorder()
+    r.finish(rec)
+    o = oracle_ref.Rasterizer()
+    o.fill(v, p, w, xf)
+    ref = o.finish()
+    tiles = sum(1 for e in ref if e[0] == "tile")
+    print(f"[api-large-fill] events={len(ref)} tiles={tiles}", flush=True)
+    assert tiles > 32 * 32 // 4
+    assert len(rec.events) == len(ref)
+    for a, c in zip(rec.events, ref):
+        assert a[:3] == c[:3]
+        if a[0] == "tile":
+            assert np.array_equal(a[3], c[3])
+        else:
+            assert a[3] == c[3]
+
+
 def test_rasterizer_api_raw_commands_and_multiple_fills(gpu_ctx):
     """move_to/line_to/command (untransformed) and fill (transformed)
     segments accumulate into one path, as in the reference."""
