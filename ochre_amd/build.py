@@ -20,6 +20,8 @@ SOURCES = ["plain.hip", "batched.hip", "stroke.hip", "atlas.hip", "capi.hip"]
 
 # -ffp-contract=off: the reference's f32 arithmetic is never fused into FMAs.
 # -packed-fp32-ops: keep the compiler from emitting packed f32 VALU ops.
+# -fno-strict-aliasing: LDS accumulator slots are read and written both as
+# 64-bit integers and, in their low words, as f32.
 FLAGS = [
     f"--offload-arch={ARCH}",
     "-O3",
@@ -28,6 +30,7 @@ FLAGS = [
     "-shared",
     "-ffp-contract=off",
     "-fno-fast-math",
+    "-fno-strict-aliasing",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall",
     "-Wno-unused-function",

@@ -1222,6 +1222,20 @@ __global__ __launch_bounds__(RT_NT) void k_row_tiles(RowTileArgs a) {
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), wid = tid / WAVE;
     const uint32_t xmask = (1u << a.xbits) - 1u;
     const uint64_t lt = lanemask_lt();
+    if constexpr (EMIT) {
+        // The window sums start at zero: cleared here once, and after each
+        // window the quantisation clears the slots it read (runs only add into
+        // their own, occupied, tiles).
+        constexpr int n16 = (int)sizeof(s.acc_a) / 16;  // 16 bytes at a time
+        static_assert(sizeof(s.acc_a) % 16 == 0, "16-byte clears");
+        float4* za = reinterpret_cast<float4*>(s.acc_a);
+        float4* zh = reinterpret_cast<float4*>(s.acc_h);
+        for (int k = tid; k < n16; k += RT_NT) {
+            za[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            zh[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+        __syncthreads();
+    }
     for (int64_t row = blockIdx.x; row < a.n_rows; row += gridDim.x) {
         const PathGrid G = a.grid[a.row_path[row]];
         const int32_t r = (int32_t)(row - (int64_t)G.row_base);
@@ -1263,18 +1277,6 @@ __global__ __launch_bounds__(RT_NT) void k_row_tiles(RowTileArgs a) {
             const bool fx = G.w < 8192 && ee - eb < RT_FX_MAX_ENTRIES;  // fixed-point sums (see RowLds)
             if (tid < UW) s.tsign[tid] = 0;
             if (tid < UW / 32) s.occ[tid] = 0;
-            if constexpr (EMIT) {
-                // all the window's sums start at zero (the runs add into them
-                // before the occupied tiles are known)
-                constexpr int n16 = (int)sizeof(s.acc_a) / 16;  // the sums are cleared 16 bytes at a time
-                static_assert(sizeof(s.acc_a) % 16 == 0, "16-byte clears");
-                float4* za = reinterpret_cast<float4*>(s.acc_a);
-                float4* zh = reinterpret_cast<float4*>(s.acc_h);
-                for (int k = tid; k < n16; k += RT_NT) {
-                    za[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                    zh[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                }
-            }
             __syncthreads();
             // one pass over the window's entries, a batch at a time
             uint32_t e0 = eb + (uint32_t)tid;
@@ -1381,6 +1383,8 @@ __global__ __launch_bounds__(RT_NT) void k_row_tiles(RowTileArgs a) {
                         const float av = fx ? fx_to_f(s.acc_a[base + k]) : fa[2 * (base + k)];
                         const float v = (cin + fh[2 * (base + k)]) + av;
                         word |= (uint32_t)fminf(fabsf(v) * 256.0f, 255.0f) << (8 * k);
+                        s.acc_a[base + k] = 0;  // cleared for the next window
+                        s.acc_h[base + k] = 0;
                     }
                     const uint64_t ti = (uint64_t)row_tiles + nt_w + rk;
                     reinterpret_cast<uint32_t*>(a.tile_alpha + ti * 64)[quad] = word;
