@@ -509,10 +509,48 @@ namespace {
 int verb_npts(uint8_t v) { return ochre::verb_points(v); }
 
 struct DevPath {
-    DBuf verbs, points, weights, seg_voff, seg_poff, seg_woff, seg_xf, seg_raw, seg_path, path_seg;
-    DBuf s_verbs, s_points, s_weights, s_voff, s_poff, s_woff, s_width;
+    DBuf s_verbs, s_points, s_weights, s_voff, s_poff, s_woff, s_width, s_xf;
     DBuf tile_xy, tile_alpha, span_xy, span_w, pts, ptc, pss, psc;
+    // the path's arrays, packed into one pinned host buffer and sent with one copy
+    void* stage_host = nullptr;
+    size_t stage_cap = 0;
+    DBuf stage_dev;
+    ~DevPath() {
+        if (stage_host) (void)hipHostFree(stage_host);
+    }
 };
+
+// Packs host arrays at 256-byte aligned offsets of dp's pinned staging buffer
+// and copies them to the device in one transfer; dev[i] receives array i's
+// device address.
+struct HostPiece {
+    const void* p;
+    size_t bytes;
+};
+static int upload_packed(DevPath* dp, const HostPiece* pc, int n, void** dev, hipStream_t s) {
+    size_t off[16];
+    size_t total = 0;
+    for (int i = 0; i < n; i++) {
+        off[i] = total;
+        total += (std::max<size_t>(pc[i].bytes, 16) + 255) & ~(size_t)255;
+    }
+    if (total > dp->stage_cap) {
+        if (dp->stage_host) (void)hipHostFree(dp->stage_host);
+        dp->stage_host = nullptr;
+        dp->stage_cap = 0;
+        const size_t cap = std::max<size_t>(total, 1 << 16);
+        if (hipHostMalloc(&dp->stage_host, cap) != hipSuccess) return OCHRE_ERR_OUT_OF_MEMORY;
+        dp->stage_cap = cap;
+    }
+    hipError_t e = dp->stage_dev.ensure(total);
+    if (e != hipSuccess) return map_hip_error(e);
+    for (int i = 0; i < n; i++) {
+        if (pc[i].bytes) memcpy(static_cast<uint8_t*>(dp->stage_host) + off[i], pc[i].p, pc[i].bytes);
+        dev[i] = static_cast<uint8_t*>(dp->stage_dev.p) + off[i];
+    }
+    e = hipMemcpyAsync(dp->stage_dev.p, dp->stage_host, total, hipMemcpyHostToDevice, s);
+    return e == hipSuccess ? 0 : map_hip_error(e);
+}
 
 template <typename T>
 int upload(DBuf& b, const std::vector<T>& v, hipStream_t s) {
@@ -536,7 +574,7 @@ int expand_strokes(ochre_context* ctx, ochre_rasterizer* r, DevPath* dp, hipStre
     if ((st = upload(dp->s_verbs, r->s_verbs, s)) || (st = upload(dp->s_points, r->s_points, s)) ||
         (st = upload(dp->s_weights, r->s_weights, s)) || (st = upload(dp->s_voff, r->s_voff, s)) ||
         (st = upload(dp->s_poff, r->s_poff, s)) || (st = upload(dp->s_woff, r->s_woff, s)) ||
-        (st = upload(dp->s_width, r->s_width, s)) || (st = upload(dp->seg_xf, xf, s)))
+        (st = upload(dp->s_width, r->s_width, s)) || (st = upload(dp->s_xf, xf, s)))
         return st;
     SegInput si;
     si.n_segs = K;
@@ -547,7 +585,7 @@ int expand_strokes(ochre_context* ctx, ochre_rasterizer* r, DevPath* dp, hipStre
     si.seg_verb_off = dp->s_voff.as<int64_t>();
     si.seg_point_off = dp->s_poff.as<int64_t>();
     si.seg_weight_off = dp->s_woff.as<int64_t>();
-    si.seg_xf = dp->seg_xf.as<Xform>();
+    si.seg_xf = dp->s_xf.as<Xform>();
     si.n_verbs = (int64_t)r->s_verbs.size();
     si.n_points = (int64_t)r->s_points.size() / 2;
     si.n_weights = (int64_t)r->s_weights.size();
@@ -723,25 +761,31 @@ int ochre_rasterizer_finish(ochre_rasterizer* r, const ochre_tile_builder* build
         const int64_t nseg = (int64_t)r->seg_xf.size();
         std::vector<int32_t> seg_path((size_t)nseg, 0);
         std::vector<int64_t> path_seg{0, nseg};
-        if ((st = upload(dp->verbs, r->verbs, s)) || (st = upload(dp->points, r->points, s)) ||
-            (st = upload(dp->weights, r->weights, s)) || (st = upload(dp->seg_voff, r->seg_verb_off, s)) ||
-            (st = upload(dp->seg_poff, r->seg_point_off, s)) || (st = upload(dp->seg_woff, r->seg_weight_off, s)) ||
-            (st = upload(dp->seg_xf, r->seg_xf, s)) || (st = upload(dp->seg_raw, r->seg_raw, s)) ||
-            (st = upload(dp->seg_path, seg_path, s)) || (st = upload(dp->path_seg, path_seg, s)))
-            break;
+        const HostPiece pc[10] = {{r->verbs.data(), r->verbs.size()},
+                                  {r->points.data(), r->points.size() * sizeof(float)},
+                                  {r->weights.data(), r->weights.size() * sizeof(float)},
+                                  {r->seg_verb_off.data(), r->seg_verb_off.size() * sizeof(int64_t)},
+                                  {r->seg_point_off.data(), r->seg_point_off.size() * sizeof(int64_t)},
+                                  {r->seg_weight_off.data(), r->seg_weight_off.size() * sizeof(int64_t)},
+                                  {r->seg_xf.data(), r->seg_xf.size() * sizeof(Xform)},
+                                  {r->seg_raw.data(), r->seg_raw.size()},
+                                  {seg_path.data(), seg_path.size() * sizeof(int32_t)},
+                                  {path_seg.data(), path_seg.size() * sizeof(int64_t)}};
+        void* dv[10];
+        if ((st = upload_packed(dp, pc, 10, dv, s))) break;
         SegInput si;
         si.n_segs = nseg;
         si.n_paths = 1;
-        si.verbs = dp->verbs.as<uint8_t>();
-        si.points = dp->points.as<float>();
-        si.weights = dp->weights.as<float>();
-        si.seg_verb_off = dp->seg_voff.as<int64_t>();
-        si.seg_point_off = dp->seg_poff.as<int64_t>();
-        si.seg_weight_off = dp->seg_woff.as<int64_t>();
-        si.seg_xf = dp->seg_xf.as<Xform>();
-        si.seg_raw = dp->seg_raw.as<uint8_t>();
-        si.seg_path = dp->seg_path.as<int32_t>();
-        si.path_seg_off = dp->path_seg.as<int64_t>();
+        si.verbs = static_cast<const uint8_t*>(dv[0]);
+        si.points = static_cast<const float*>(dv[1]);
+        si.weights = static_cast<const float*>(dv[2]);
+        si.seg_verb_off = static_cast<const int64_t*>(dv[3]);
+        si.seg_point_off = static_cast<const int64_t*>(dv[4]);
+        si.seg_weight_off = static_cast<const int64_t*>(dv[5]);
+        si.seg_xf = static_cast<const Xform*>(dv[6]);
+        si.seg_raw = static_cast<const uint8_t*>(dv[7]);
+        si.seg_path = static_cast<const int32_t*>(dv[8]);
+        si.path_seg_off = static_cast<const int64_t*>(dv[9]);
         si.n_verbs = (int64_t)r->verbs.size();
         si.n_points = (int64_t)r->points.size() / 2;
         si.n_weights = (int64_t)r->weights.size();
