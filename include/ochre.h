@@ -55,7 +55,8 @@ enum ochre_status {
     OCHRE_ERR_OUT_OF_MEMORY = -2,
     OCHRE_ERR_DEVICE = -3,
     OCHRE_ERR_CAPACITY = -4,      /* output arrays too small; required sizes reported */
-    OCHRE_ERR_FLATTEN_LIMIT = -5, /* a curve needed > 65536 lines or conic depth > 32
+    OCHRE_ERR_FLATTEN_LIMIT = -5, /* a curve needed 65536 or more lines, a conic more than 32
+                                     levels of subdivision, or a line more than 2^20 DDA steps
                                      (inputs on which the reference does not terminate) */
     OCHRE_ERR_NO_DEVICE = -6,
     OCHRE_ERR_TOO_LARGE = -7      /* one path exceeds the per-pass limits (internal 32-bit indexing) */
@@ -156,8 +157,10 @@ void ochre_context_destroy(ochre_context* ctx);
 /* The five-kernel pipeline: (a) flatten, (b) DDA binning + device radix
  * sort, (c)-(e) per-tile accumulation, row scan and compaction.  Paths are
  * output contiguously in path order.  `stream` is a hipStream_t (NULL =
- * default stream).  Synchronises the stream to size its scratch; outputs are
- * complete when the stream is idle. */
+ * default stream).  Synchronises the stream to size its scratch and, at the
+ * end, to read the call's error flags, so the outputs are complete when it
+ * returns.  An internal invariant broken on the device (a run replay past its
+ * step bound, a run outside its path's grid) returns OCHRE_ERR_DEVICE. */
 int ochre_rasterize_plain(ochre_context* ctx, const ochre_path_batch* in, ochre_batch_output* out,
                           void* stream);
 
@@ -171,7 +174,7 @@ int ochre_rasterize_batched(ochre_context* ctx, const ochre_path_batch* in, ochr
                             void* stream);
 
 /* Rasterizer::stroke for every path of a batch (widths: [n_paths], device
- * memory): flattening and stroke expansion on the device (one lane per path),
+ * memory): flattening and stroke expansion on the device (one wave per path),
  * then the plain or batched fill of the outlines with the paths' transforms.
  * Returns OCHRE_ERR_INVALID_ARG for verbs that do not match their points or
  * weights and OCHRE_ERR_FLATTEN_LIMIT for a curve the reference would not

@@ -124,6 +124,7 @@ struct Context {
     DBuf lines, line_path, line_cnt, line_off, dda_blk;
     DBuf path_box, path_grid, path_rows;
     DBuf counters, totals;
+    DBuf call_error;  // emit-phase error bits of the current plain call (all passes)
     DBuf keys[2], vals[2], records, rs_counts;
     DBuf grp_begin, grp_key, grp_hasrun, grp_sign;
     DBuf row_gbegin, row_gend, row_sign, row_wb, row_path, row_cnt, row_off;

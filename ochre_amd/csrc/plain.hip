@@ -1448,6 +1448,13 @@ struct HostTotals {
 // Internal status of one pass: the input is over the per-pass limits.
 constexpr int ST_SPLIT = -100;
 
+// Folds a pass's error bits into the call's word: the emit kernels report a
+// replay overrun (ERR_REPLAY) after the host's last read of the pass's
+// counters, which the next pass clears.
+__global__ void k_fold_error(const Counters* cnt, uint32_t* call_error) {
+    if (threadIdx.x == 0 && cnt->error) atomicOr(call_error, cnt->error);
+}
+
 struct StoreBucketStart {
     uint32_t* beg;
     uint32_t* cur;
@@ -1697,6 +1704,7 @@ static int run_plain_pass(Context* ctx, const SegInput& in, OutputDesc& out, hip
         hipLaunchKernelGGL(k_path_out, dim3(grid_for(NP, 256)), dim3(256), 0, s, grid, NP, ra.row_off,
                            (const uint64_t*)&tot[4], out.path_tile_start, out.path_tile_count, out.path_span_start,
                            out.path_span_count, out.tile_xy, out.tile_alpha, out.tile_base, out.span_base);
+        hipLaunchKernelGGL(k_fold_error, dim3(1), dim3(64), 0, s, cnt, ctx->call_error.as<uint32_t>());
         HIPCHK(hipGetLastError());
         if (ctx->timing) {
             HIPCHK(hipEventSynchronize(ctx->ev[1]));
@@ -1811,6 +1819,7 @@ static int run_plain_pass(Context* ctx, const SegInput& in, OutputDesc& out, hip
     hipLaunchKernelGGL(k_path_out, dim3(grid_for(NP, 256)), dim3(256), 0, s, grid, NP, ta.row_off,
                        (const uint64_t*)&tot[4], out.path_tile_start, out.path_tile_count, out.path_span_start,
                        out.path_span_count, out.tile_xy, out.tile_alpha, out.tile_base, out.span_base);
+    hipLaunchKernelGGL(k_fold_error, dim3(1), dim3(64), 0, s, cnt, ctx->call_error.as<uint32_t>());
     HIPCHK(hipGetLastError());
     if (ctx->timing) {
         HIPCHK(hipEventSynchronize(ctx->ev[1]));
@@ -1834,9 +1843,24 @@ __global__ void k_rebase_offsets(const int64_t* voff, const int64_t* poff, const
     dst[2 * (n + 1) + i] = woff[p0 + i] - woff[p0];
 }
 
+// The emit-phase error bits of a call (folded in by k_fold_error after each
+// pass), read back once the call's work is queued: a replay overrun makes the
+// call fail with -3 instead of returning truncated alpha.
+static int check_call_error(Context* ctx, int st, hipStream_t s) {
+    if (st != 0 && st != -4) return st;
+    uint32_t* h = reinterpret_cast<uint32_t*>(reinterpret_cast<uint8_t*>(ctx->host_pinned) + 2048);
+    HIPCHK(hipMemcpyAsync(h, ctx->call_error.p, 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (*h & (ERR_BBOX | ERR_REPLAY)) return -3;
+    return st;
+}
+
 int run_plain(Context* ctx, const SegInput& in, OutputDesc& out, hipStream_t s, bool unordered) {
+    HIPCHK(ctx->call_error.ensure(256));
+    if (!ctx->host_pinned) HIPCHK(hipHostMalloc(&ctx->host_pinned, 4096));
+    fill<uint32_t>(ctx->call_error.as<uint32_t>(), 1, 0u, s);
     int st = run_plain_pass(ctx, in, out, s, unordered);
-    if (st != ST_SPLIT) return st;
+    if (st != ST_SPLIT) return check_call_error(ctx, st, s);
     // Paths are independent Rasterizers: rasterise path ranges one after
     // another, each range's tiles and spans following the previous range's.
     if (in.path_seg_off || in.seg_path || in.n_paths < 2) return -7;
@@ -1916,7 +1940,7 @@ int run_plain(Context* ctx, const SegInput& in, OutputDesc& out, hipStream_t s, 
     }
     out.n_tiles = T;
     out.n_spans = S;
-    return (room && T <= out.tile_capacity && S <= out.span_capacity) ? 0 : -4;
+    return check_call_error(ctx, (room && T <= out.tile_capacity && S <= out.span_capacity) ? 0 : -4, s);
 }
 
 }  // namespace ochre

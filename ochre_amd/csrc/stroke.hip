@@ -8,9 +8,10 @@
 //            polyline with miter joins (a bevel where the miter would reach
 //            beyond twice the half width);
 // and the outline batch then goes through the fill pipelines unchanged.
-// Each pass runs one lane per path, once to count and once to write, with a
-// device scan between; a path's outline is produced by one lane in order, so
-// its command sequence is the reference's exactly.
+// Each pass runs one wave per path, once to count and once to write, with a
+// device scan between; the wave works through a path 64 commands (or
+// indices) at a time and places each lane's output at positions from a wave
+// scan, so the path's outline command sequence is the reference's exactly.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>

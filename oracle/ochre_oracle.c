@@ -18,6 +18,11 @@
 #define TOLERANCE 0.1f         /* rasterizer.rs:6 */
 #define FLATTEN_LIMIT (1 << 16) /* guard: the reference loops forever when dt == 0 */
 #define CONIC_DEPTH_LIMIT 32    /* guard: the reference recurses without bound */
+/* guard: increments per line_to.  A line whose crossing step is 0 (a finite
+ * end point joined to an infinite one, e.g. the close of M(inf,3) L(40,5)
+ * L(20,40)) never reaches t = 1 in the reference; the device walks give up
+ * after the same number of steps with the same status. */
+#define WALK_LIMIT (1u << 20)
 
 /* ------------------------------------------------------------------ geom.rs */
 
@@ -139,6 +144,7 @@ void orc_line_to(orc_rasterizer* r, float px, float py) {
         }
         float x_step = fabsf(dtdx);
         float y_step = fabsf(dtdy);
+        uint32_t steps = 0;
 
         for (;;) {
             float t0 = rmax(row_t0, col_t0);
@@ -182,6 +188,10 @@ void orc_line_to(orc_rasterizer* r, float px, float py) {
             }
 
             if (row_t0 == 1.0f || col_t0 == 1.0f) break;
+            if (++steps >= WALK_LIMIT) {
+                r->status |= ORC_ERR_FLATTEN_LIMIT;
+                break;
+            }
         }
     }
     r->last = point;

@@ -13,7 +13,9 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
-ORACLE_SO = os.path.join(ORACLE_DIR, "libochre_oracle.so")
+# OCHRE_ORACLE_SO selects another build of the same sources (the sanitizer
+# build, oracle/_san/libochre_oracle.so, in tests/test_oracle_sanitize.py).
+ORACLE_SO = os.environ.get("OCHRE_ORACLE_SO") or os.path.join(ORACLE_DIR, "libochre_oracle.so")
 
 _lib = None
 

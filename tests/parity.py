@@ -41,3 +41,61 @@ def _first_row_diff(what, a, b, off):
     i = int(rows[0])
     p = int(np.searchsorted(off, i, side="right") - 1)
     return f"{what}: first difference at {i} (path {p}): got {a[i]} ref {b[i]}"
+
+
+def path_digests(txy, ta, sxy, sw, pto, pso):
+    """Per-path SHA-256 digests of an output in TileBuilder order: `sets`
+    covers the tile coordinates, span coordinates and span widths (the
+    integer part of the contract), `alpha` the tiles' alpha bytes.  Returns a
+    list of (n_tiles, n_spans, sets_hex, alpha_hex)."""
+    import hashlib
+    txy = np.ascontiguousarray(txy, np.int16)
+    ta = np.ascontiguousarray(ta, np.uint8)
+    sxy = np.ascontiguousarray(sxy, np.int16)
+    sw = np.ascontiguousarray(sw, np.uint16)
+    out = []
+    for i in range(len(pto) - 1):
+        t0, t1, s0, s1 = int(pto[i]), int(pto[i + 1]), int(pso[i]), int(pso[i + 1])
+        h = hashlib.sha256()
+        h.update(txy[t0:t1].tobytes())
+        h.update(b"|")
+        h.update(sxy[s0:s1].tobytes())
+        h.update(b"|")
+        h.update(sw[s0:s1].tobytes())
+        a = hashlib.sha256(ta[t0:t1].tobytes())
+        out.append((t1 - t0, s1 - s0, h.hexdigest(), a.hexdigest()))
+    return out
+
+
+def input_digests(batch):
+    """Per-path SHA-256 of a batch's input (verbs, points, weights and
+    transform bytes): shows that a generator reproduced the paths a fixture
+    was made from."""
+    import hashlib
+    out = []
+    for i in range(batch.n_paths):
+        v, p, w, xf = batch.path(i)
+        h = hashlib.sha256()
+        for a in (v, p, w, xf):
+            h.update(np.ascontiguousarray(a).tobytes())
+            h.update(b"|")
+        out.append(h.hexdigest())
+    return out
+
+
+def basic_rs_text(events):
+    """TileBuilder events as the reference's examples/basic.rs:6-19 prints
+    them: "tile at (x, y):" and eight rows of "{:3} " alphas indented by two
+    spaces, or "span at (x, y), width w"."""
+    lines = []
+    for e in events:
+        if e[0] == "tile":
+            _, x, y, a = e
+            a = np.asarray(a, np.uint8).reshape(8, 8)
+            lines.append(f"tile at ({x}, {y}):")
+            for row in a:
+                lines.append("  " + "".join(f"{int(v):3d} " for v in row))
+        else:
+            _, x, y, w = e
+            lines.append(f"span at ({x}, {y}), width {w}")
+    return "\n".join(lines) + "\n"

@@ -46,24 +46,22 @@ def test_forward_coverage_matches_alpha(gpu_ctx):
     assert np.array_equal(np.minimum(np.floor(cov * 256.0), 255).astype(np.uint8), alpha)
 
 
-@pytest.mark.parametrize("n,seed", [(96, 0), (96, 1)])
-def test_adjoint_matches_f64_oracle_and_finite_differences(gpu_ctx, n, seed):
+def _check_adjoint(ctx, b, tb, fd_samples, seed):
+    """Loss and every gradient element of the device adjoint against the f64
+    oracle adjoint, and `fd_samples` seeded parameters against central finite
+    differences of the f64 forward."""
     import torch
     from ochre_amd.device import coverage_adjoint, upload
-    b = W.adjoint_glyphs(n=n, seed=100 + seed)
-    tb = perturbed(b, 0.5, 900 + seed)
-    tgt = render_target(gpu_ctx, tb)
-    loss, dp, dx = coverage_adjoint(gpu_ctx, upload(b), tgt)
+    tgt = render_target(ctx, tb)
+    loss, dp, dx = coverage_adjoint(ctx, upload(b), tgt)
     torch.cuda.synchronize()
     loss, dp, dx = loss.cpu().numpy(), dp.cpu().numpy(), dx.cpu().numpy()
-    rng = np.random.default_rng(seed)
-    worst_g = 0.0
-    checked_fd = 0
+    targets = [host_target(tgt, i) for i in range(b.n_paths)]
+    errs = []
+    worst = (0.0, None)
     for i in range(b.n_paths):
         v, p, w, xf = b.path(i)
-        t = host_target(tgt, i)
-        p64, xf64 = p.astype(np.float64), xf.astype(np.float64)
-        st, L, gp, gx, _ = O.path_loss64(v, p64, w, xf64, t, grad=True)
+        st, L, gp, gx, _ = O.path_loss64(v, p.astype(np.float64), w, xf.astype(np.float64), targets[i], grad=True)
         assert st == 0
         # f32 coverage against f64: at canvas coordinates up to 1024 px an f32
         # point carries ~6e-5 px of rounding, so an edge pixel's coverage
@@ -71,35 +69,64 @@ def test_adjoint_matches_f64_oracle_and_finite_differences(gpu_ctx, n, seed):
         # d(cov - t)^2 = 2 (cov - t) dcov) by ~1e-3 relative at most
         assert abs(loss[i] - L) <= 1e-3 * max(L, 1.0), (i, loss[i], L)
         a, e = b.point_off[i], b.point_off[i + 1]
-        scale = max(np.abs(gp).max(), np.abs(gx).max(), 1.0)
-        err = max(np.abs(dp[a:e] - gp).max(), np.abs(dx[i] - gx).max()) / scale
-        worst_g = max(worst_g, err)
-        assert err <= 1e-3, (i, err)
-        # central finite differences of the f64 forward (h = 1e-3 px) on
-        # sampled parameters, up to 3 tries per path for one without a kink
-        # in [x - h, x + h] (a kink: the one-sided quotients disagree by more
-        # than the curvature of a smooth loss explains at this h)
-        for _ in range(3):
-            if checked_fd >= 64:
-                break
-            k = int(rng.integers(0, p64.size + 6))
-            if k < p64.size:
-                h = 1e-3 / float(xf64[0])
-                f = lambda z: O.path_loss64(v, z.reshape(-1, 2), w, xf64, t)[1]
-                x0, gk = p64.reshape(-1).copy(), float(dp[a:e].reshape(-1)[k])
-            else:
-                k -= p64.size
-                h = 1e-3 / 32.0 if k < 4 else 1e-3
-                f = lambda z: O.path_loss64(v, p64, w, z, t)[1]
-                x0, gk = xf64.copy(), float(dx[i, k])
-            xp, xm = x0.copy(), x0.copy()
-            xp[k] += h
-            xm[k] -= h
-            lp, lm, l0 = f(xp), f(xm), f(x0)
-            c, r, l = (lp - lm) / (2 * h), (lp - l0) / h, (l0 - lm) / h
-            if abs(r - l) <= 2e-2 * max(abs(c), 1.0):
-                assert abs(gk - c) <= 1e-3 * max(abs(c), 1.0), (i, k, gk, c)
-                checked_fd += 1
-                break
-    print(f"adjoint n={n}: worst gradient rel err vs f64 adjoint {worst_g:.2e}, fd checks {checked_fd}", flush=True)
-    assert checked_fd >= 32
+        # per parameter: |g - g64| <= 1e-3 * max(|g64|, 1)
+        got = np.concatenate([dp[a:e].reshape(-1), dx[i]])
+        ref = np.concatenate([gp.reshape(-1), gx])
+        rel = np.abs(got - ref) / np.maximum(np.abs(ref), 1.0)
+        errs.append(rel)
+        k = int(np.argmax(rel))
+        if rel[k] > worst[0]:
+            worst = (float(rel[k]), (i, k, float(got[k]), float(ref[k])))
+    errs = np.concatenate(errs)
+    print(f"adjoint {b.name}: {errs.size} gradient elements, per-parameter rel err vs f64 adjoint: "
+          f"median {np.median(errs):.1e}, p99.9 {np.quantile(errs, 0.999):.1e}, max {worst[0]:.1e} at {worst[1]}",
+          flush=True)
+    assert worst[0] <= 1e-3, worst
+    # central finite differences of the f64 forward (h = 1e-3 px) on seeded
+    # (path, parameter) samples; a sample whose [x - h, x + h] holds a kink
+    # (the one-sided quotients disagree by more than the curvature of a smooth
+    # loss explains at this h) is replaced by another draw
+    rng = np.random.default_rng(seed)
+    checked, draws = 0, 0
+    while checked < fd_samples and draws < 8 * fd_samples:
+        draws += 1
+        i = int(rng.integers(0, b.n_paths))
+        v, p, w, xf = b.path(i)
+        p64, xf64, t = p.astype(np.float64), xf.astype(np.float64), targets[i]
+        a = b.point_off[i]
+        k = int(rng.integers(0, p64.size + 6))
+        if k < p64.size:
+            h = 1e-3 / float(xf64[0])
+            f = lambda z: O.path_loss64(v, z.reshape(-1, 2), w, xf64, t)[1]  # noqa: E731
+            x0, gk = p64.reshape(-1).copy(), float(dp[a:a + p64.shape[0]].reshape(-1)[k])
+        else:
+            k -= p64.size
+            h = 1e-3 / 32.0 if k < 4 else 1e-3
+            f = lambda z: O.path_loss64(v, p64, w, z, t)[1]  # noqa: E731
+            x0, gk = xf64.copy(), float(dx[i, k])
+        xp, xm = x0.copy(), x0.copy()
+        xp[k] += h
+        xm[k] -= h
+        lp, lm, l0 = f(xp), f(xm), f(x0)
+        c, r, l = (lp - lm) / (2 * h), (lp - l0) / h, (l0 - lm) / h
+        if abs(r - l) <= 2e-2 * max(abs(c), 1.0):
+            assert abs(gk - c) <= 1e-3 * max(abs(c), 1.0), (i, k, gk, c)
+            checked += 1
+    print(f"adjoint {b.name}: {checked} finite-difference checks in {draws} draws", flush=True)
+    assert checked >= fd_samples
+
+
+def test_adjoint_configured_batch(gpu_ctx):
+    """The differentiable configuration itself: 4,096 paths of 16 cubics
+    (32-64 px) on 1024x1024 against a target with control points perturbed by
+    N(0, 0.5 px); every gradient element against the f64 oracle adjoint and 64
+    finite-difference samples."""
+    b = W.adjoint_glyphs()
+    assert b.n_paths == 4096
+    _check_adjoint(gpu_ctx, b, perturbed(b, 0.5, 900), fd_samples=64, seed=4)
+
+
+def test_adjoint_side_batch(gpu_ctx):
+    """Another seed of the same generator (96 paths)."""
+    b = W.adjoint_glyphs(n=96, seed=100)
+    _check_adjoint(gpu_ctx, b, perturbed(b, 0.5, 901), fd_samples=32, seed=5)

@@ -283,3 +283,58 @@ def test_batched_params_and_graph_replays(gpu_ctx):
         lib.ochre_context_set_dense_params(gpu_ctx.h, 1, 4, 4)
         lib.ochre_context_set_dense_small(gpu_ctx.h, 0)
         lib.ochre_context_set_dense_large(gpu_ctx.h, 1)
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_non_terminating_line_returns_flatten_limit(gpu_ctx, mode):
+    """The close of M(inf,3) L(40,5) L(20,40) never reaches t = 1 in the
+    reference's DDA.  Every mode reaches the plain pipeline's count walk (the
+    dense and ordered batched kernels hand a path with non-finite points on
+    before walking), which stops after 2^20 steps: OCHRE_ERR_FLATTEN_LIMIT,
+    the status the oracle's guard gives for the same path."""
+    from ochre_amd import _lib
+    from ochre_amd.device import run_to_host
+    b = W.from_paths([([W.MOVE, W.LINE, W.LINE], np.array([(np.inf, 3), (40, 5), (20, 40)], np.float32))],
+                     W.IDENTITY)
+    assert oracle_ref.run_batch(b).status == 2
+    print(f"[{mode}] non-terminating close", flush=True)
+    with pytest.raises(_lib.OchreError) as e:
+        run_to_host(gpu_ctx, b, mode)
+    assert e.value.status == -5
+
+
+@pytest.mark.parametrize("mode", ["plain", "batched"])
+def test_run_to_run_determinism(gpu_ctx, mode):
+    """The exact modes are reproducible: two runs of the same batch give
+    byte-identical per-path output (the ordered modes sum each pixel in
+    emission order, rasterizer.rs:209-228).  The batch mixes glyphs, fuzz paths
+    and a blob the batched call hands to the plain pipeline.  (The dense mode
+    sums in arrival order and is not reproducible bit for bit; it is checked
+    against the oracle within 1 LSB instead.)"""
+    from ochre_amd.device import run_to_host
+    blobs = W.star_blobs()
+    small = blobs.subset(np.argsort(np.diff(blobs.verb_off))[:1])
+    b = W.concat([W.glyph_run(n=3000, seed=21), W.random_paths(300, 22), small])
+    b.name = "determinism"
+    print(f"[{mode}] determinism: {b.n_paths} paths", flush=True)
+    first = run_to_host(gpu_ctx, b, mode)
+    second = run_to_host(gpu_ctx, b, mode)
+    for x, y in zip(first, second):
+        assert x.dtype == y.dtype and x.shape == y.shape and x.tobytes() == y.tobytes()
+
+
+def test_readme_fixture_through_the_rasterizer_api(gpu_ctx):
+    """The drop-in Rasterizer on the README path prints, in the reference's
+    examples/basic.rs format, exactly the committed fixture
+    tests/golden/readme_path_basic.txt."""
+    import os
+
+    from ochre_amd import EventRecorder, Rasterizer
+    from parity import basic_rs_text
+    v, p, w, xf = W.readme_path().path(0)
+    r = Rasterizer()
+    r.fill_arrays(v, p, w, xf)
+    rec = EventRecorder()
+    r.finish(rec)
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "readme_path_basic.txt")
+    assert basic_rs_text(rec.events) == open(path).read()

@@ -346,3 +346,121 @@ def test_batch_runner_matches_single_rasterizers():
         for k, e in enumerate(ev):
             assert tuple(res.tile_xy[t0 + k]) == (e[1], e[2])
             assert np.array_equal(res.tile_alpha[t0 + k], e[3])
+
+
+# ------------------------------------------------------- guards and Q4 pins
+
+def test_non_terminating_line_stops_at_walk_limit():
+    """The close of M(inf,3) L(40,5) L(20,40) joins (20,40) to (inf,3): its
+    crossing step |1/dx| is 0, so the reference's DDA (rasterizer.rs:97-136)
+    never reaches t = 1.  The oracle stops after 2^20 increments with the
+    flatten-limit status, as the device walks do, instead of growing its
+    increment list until the allocation fails."""
+    b = W.from_paths([([M, L, L], np.array([(np.inf, 3), (40, 5), (20, 40)], np.float32))], W.IDENTITY)
+    res = O.run_batch(b)
+    assert res.status == 2  # ORC_ERR_FLATTEN_LIMIT
+    assert res.stats["increments"] <= (1 << 20) + 64
+
+
+def _events(verbs, pts):
+    r = O.Rasterizer()
+    r.fill(verbs, pts)
+    return r.finish()
+
+
+def test_zero_length_step_at_a_tile_corner_adds_tiles():
+    """Hand-derived: the segment (7.5,7.5)-(8.5,8.5) crosses the pixel corner
+    (8,8), which is also a tile corner.  dtdx = dtdy = 1, so row_t1 = col_t1
+    = 0.5: the tie steps x first (rasterizer.rs:108), and the next increment,
+    at pixel (8,7) with t0 = t1 = 0.5, has zero length.  Its close back to
+    (7.5,7.5) does the same at pixel (7,8).  Increments: (7,7) area 1/8 height
+    1/2, (8,7) zero, (8,8) area 3/8 height 1/2; then (8,8) area -3/8 height
+    -1/2, (7,8) zero, (7,7) area -1/8 height -1/2.  Every pixel sums to 0, so
+    the output is four all-zero tiles in row-major order; the two off-diagonal
+    ones exist only because of the zero-length increments.  Crossings:
+    (tx 1, ty 0, +1) and (tx 0, ty 0, -1) sum to 0, and no row has a gap, so
+    there are no spans."""
+    ev = _events([M, L], [(7.5, 7.5), (8.5, 8.5)])
+    assert [e[:3] for e in ev] == [("tile", 0, 0), ("tile", 8, 0), ("tile", 0, 8), ("tile", 8, 8)]
+    assert all(not e[3].any() for e in ev)
+
+
+def test_negative_start_on_an_integer_coordinate_adds_a_tile():
+    """Hand-derived: the horizontal segment (8,4.5) -> (6.5,4.5) starts on
+    x = 8 and moves left, so col_t1 = dtdx * (8 - 8) = -0 (rasterizer.rs:91-92)
+    and the first increment, at pixel (8,4) with t0 = 0, t1 = -0, has zero
+    length.  The rest (pixels (7,4), (6,4)) and the close back to (8,4.5)
+    (pixels (6,4), (7,4); its last step consumes t = 1 and ends the walk
+    before pixel (8,4)) have height 0.  So the output is two all-zero tiles,
+    (0,0) and (8,0); tile (8,0) comes from the -0 increment alone."""
+    ev = _events([M, L], [(8.0, 4.5), (6.5, 4.5)])
+    assert [e[:3] for e in ev] == [("tile", 0, 0), ("tile", 8, 0)]
+    assert all(not e[3].any() for e in ev)
+
+
+def _seg_tiles(a, b):
+    """Tiles whose open interior the f64 segment a-b passes through
+    (Liang-Barsky clipping against each candidate tile square)."""
+    out = set()
+    lo = np.floor(np.minimum(a, b) / 8).astype(int)
+    hi = np.floor(np.maximum(a, b) / 8).astype(int)
+    d = b - a
+    for ty in range(lo[1], hi[1] + 1):
+        for tx in range(lo[0], hi[0] + 1):
+            t0, t1 = 0.0, 1.0
+            ok = True
+            for k, (mn, mx) in enumerate(((8 * tx, 8 * tx + 8), (8 * ty, 8 * ty + 8))):
+                if d[k] == 0.0:
+                    if not (mn < a[k] < mx):
+                        ok = False
+                    continue
+                u0, u1 = (mn - a[k]) / d[k], (mx - a[k]) / d[k]
+                t0, t1 = max(t0, min(u0, u1)), min(t1, max(u0, u1))
+            if ok and t1 - t0 > 1e-12:
+                out.add((tx, ty))
+    return out
+
+
+def _winding(poly, x, y):
+    """Nonzero winding number of the closed f64 polygon around (x, y)."""
+    w = 0
+    n = len(poly)
+    for i in range(n):
+        (x0, y0), (x1, y1) = poly[i], poly[(i + 1) % n]
+        if y0 <= y < y1 and (x1 - x0) * (y - y0) - (x - x0) * (y1 - y0) > 0:
+            w += 1
+        elif y1 <= y < y0 and (x1 - x0) * (y - y0) - (x - x0) * (y1 - y0) < 0:
+            w -= 1
+    return w
+
+
+def test_readme_tiles_and_spans_from_geometry():
+    """The README path's tile and span sets, derived without the DDA: the
+    tiles are those whose interior the flattened outline passes through (f64
+    clipping; the path's coordinates hit no tile corner), and a gap between
+    two tiles of a row is a span exactly when the outline winds around the
+    gap's pixels (f64 winding number at the first gap pixel's centre).  This
+    pins the 100 tiles and 23 spans the oracle emits."""
+    b = W.readme_path()
+    v, p, w, xf = b.path(0)
+    st, (fv, fp, _) = O.flatten_path(v, p, w, 0.1)
+    assert st == 0
+    assert fv[0] == M and (fv[1:-1] == L).all() and fv[-1] == Z  # one contour: Move, 70 Lines, Close
+    poly = fp.astype(np.float64)
+    assert np.array_equal(poly[0], poly[-1])  # the cubic ends exactly at the start
+    poly = poly[:-1]
+    tiles = set()
+    for i in range(len(poly)):
+        tiles |= _seg_tiles(poly[i], poly[(i + 1) % len(poly)])
+    ev = _events(v, p)
+    got_tiles = [(e[1] // 8, e[2] // 8) for e in ev if e[0] == "tile"]
+    assert sorted(got_tiles, key=lambda t: (t[1], t[0])) == got_tiles
+    assert set(got_tiles) == tiles and len(tiles) == 100
+    spans = []
+    for ty in sorted({t[1] for t in tiles}):
+        row = sorted(t[0] for t in tiles if t[1] == ty)
+        for a, c in zip(row, row[1:]):
+            if c > a + 1 and _winding(poly, 8 * (a + 1) + 0.5, 8 * ty + 0.5) != 0:
+                spans.append((8 * (a + 1), 8 * ty, 8 * (c - a - 1)))
+    got_spans = [tuple(e[1:]) for e in ev if e[0] == "span"]
+    assert got_spans == spans and len(spans) == 23
