@@ -306,6 +306,59 @@ int ochre_autotune_batched(ochre_context* ctx, const ochre_path_batch* in, ochre
 int ochre_context_set_timing(ochre_context* ctx, int enabled);
 int ochre_context_timing(ochre_context* ctx, const char* kernel, double* total_ms, int64_t* launches);
 
+/* A draw list: path p is its own Rasterizer, filled (stroke_flags[p] == 0,
+ * Rasterizer::fill) or stroked with widths[p] (stroke_flags[p] == 1,
+ * Rasterizer::stroke), in path order -- the sequence of Rasterizers the
+ * demo's SVG walk finishes into one Builder (examples/svg.rs:140-156).
+ * stroke_flags and widths are device arrays of n_paths.  Errors as for
+ * ochre_stroke_*.  With every flag 0 the result equals ochre_rasterize_*;
+ * with every flag 1 it equals ochre_stroke_*. */
+int ochre_draw_plain(ochre_context* ctx, const ochre_path_batch* in, const uint8_t* stroke_flags,
+                     const float* widths, ochre_batch_output* out, void* stream);
+int ochre_draw_batched(ochre_context* ctx, const ochre_path_batch* in, const uint8_t* stroke_flags,
+                       const float* widths, ochre_batch_output* out, void* stream);
+
+/* ------------------------------------------------------------- SVG input */
+
+/* SVG ingestion (host): parses a document into the draw calls the demo makes
+ * from it (examples/svg.rs:103-164, which reads the file with usvg 0.13):
+ * for every path element (and rect, circle, ellipse, line, polyline,
+ * polygon) in document order, a fill draw when its fill is a solid colour,
+ * then a stroke draw when its stroke is a solid colour of positive width.
+ * Path data is converted as usvg does (absolute f64 MoveTo / LineTo /
+ * CurveTo / ClosePath, quadratics and arcs as cubics) and cast to f32; a
+ * draw's transform is the product of its element's and its ancestors'
+ * transforms up to the nearest group usvg keeps (opacity, clip-path, mask or
+ * filter), as the demo reads node.transform() (svg.rs:107-111).  The viewBox
+ * is reported, not applied (the demo does not apply it).  Gradients,
+ * patterns, text, images, <use> and CSS style sheets are not drawn.
+ * Returns OCHRE_ERR_INVALID_ARG when the text is not an XML document whose
+ * root element is <svg>. */
+typedef struct ochre_svg_document ochre_svg_document;
+typedef struct ochre_svg_draw {
+    int64_t path;              /* index of the draw's path (its verb and point ranges) */
+    int32_t stroke;            /* 0: Rasterizer::fill, 1: Rasterizer::stroke with `width` */
+    float width;               /* stroke width in user units */
+    uint8_t color[4];          /* rgb and opacity: the Builder's colour for the call (svg.rs:141,150) */
+    ochre_transform transform; /* applied to the path's points */
+} ochre_svg_draw;
+typedef struct ochre_svg_summary {
+    int64_t n_paths, n_verbs, n_points, n_draws;
+    int64_t n_skipped;  /* elements not drawn (not graphics, unsupported, display="none") */
+    float view_box[4];  /* min-x, min-y, width, height (0 when absent) */
+    float width, height;
+} ochre_svg_summary;
+int ochre_svg_parse(const char* text, size_t len, ochre_svg_document** out);
+int ochre_svg_info(const ochre_svg_document* doc, ochre_svg_summary* info);
+/* Paths of the document, structure-of-arrays as in ochre_path_batch (host
+ * memory owned by the document; offsets have n_paths + 1 entries). */
+const uint8_t* ochre_svg_verbs(const ochre_svg_document* doc);
+const int64_t* ochre_svg_verb_offsets(const ochre_svg_document* doc);
+const float* ochre_svg_points(const ochre_svg_document* doc);
+const int64_t* ochre_svg_point_offsets(const ochre_svg_document* doc);
+const ochre_svg_draw* ochre_svg_draws(const ochre_svg_document* doc);
+void ochre_svg_free(ochre_svg_document* doc);
+
 /* Version / build information (NUL-terminated, static storage). */
 const char* ochre_build_info(void);
 

@@ -31,6 +31,9 @@ EXPORTS = [
     "ochre_context_set_dense_params", "ochre_context_get_dense_params", "ochre_build_atlas",
     "ochre_context_set_dense_small", "ochre_context_get_dense_small", "ochre_context_set_pass_limits",
     "ochre_context_set_dense_large", "ochre_context_get_dense_large",
+    "ochre_draw_plain", "ochre_draw_batched",
+    "ochre_svg_parse", "ochre_svg_info", "ochre_svg_verbs", "ochre_svg_verb_offsets", "ochre_svg_points",
+    "ochre_svg_point_offsets", "ochre_svg_draws", "ochre_svg_free",
 ]
 
 
@@ -153,6 +156,18 @@ def _optional(lib):
                                                 ctypes.c_float, P]
         for name in ("ochre_stroke_plain", "ochre_stroke_batched"):
             getattr(lib, name).argtypes = [P, ctypes.POINTER(PathBatchC), P, ctypes.POINTER(BatchOutputC), P]
+    if hasattr(lib, "ochre_draw_plain"):
+        for name in ("ochre_draw_plain", "ochre_draw_batched"):
+            getattr(lib, name).argtypes = [P, ctypes.POINTER(PathBatchC), P, P, ctypes.POINTER(BatchOutputC), P]
+    if hasattr(lib, "ochre_svg_parse"):
+        lib.ochre_svg_parse.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(P)]
+        lib.ochre_svg_info.argtypes = [P, P]
+        for name in ("ochre_svg_verbs", "ochre_svg_verb_offsets", "ochre_svg_points", "ochre_svg_point_offsets",
+                     "ochre_svg_draws"):
+            getattr(lib, name).argtypes = [P]
+            getattr(lib, name).restype = P
+        lib.ochre_svg_free.argtypes = [P]
+        lib.ochre_svg_free.restype = None
 
 
 def check(status: int, what: str):

@@ -804,7 +804,7 @@ __global__ void k_scatter_ranges(const uint32_t* ids, uint32_t n, const int64_t*
 }
 
 struct BatchedState {
-    DBuf scratch, ascratch, ctl, deferred, dscratch, dense_ids, dense_ids2, dense_ids3;
+    DBuf scratch, ascratch, ctl, deferred, dense_ids, dense_ids2, dense_ids3;
     DBuf g_verbs, g_points, g_weights, g_voff, g_poff, g_woff, g_xf, g_ts, g_tc, g_ss, g_sc;
     int n_waves = 0;
     uint32_t gran = 1;
@@ -877,18 +877,14 @@ int run_batched(Context* ctx, const SegInput& in, OutputDesc& out, hipStream_t s
         stages[n_stages++] = D_MAIN;
         if (n_large) stages[n_stages++] = D_LARGE;
     }
-    uint8_t* dscratch = nullptr;
     uint32_t* lists[3] = {nullptr, nullptr, nullptr};
     uint32_t* counts[3] = {&ctl->dq, &ctl->dq2, &ctl->dq3};
     uint32_t* queues[3] = {&ctl->queue, &ctl->queue3, &ctl->queue4};
     FusedIn fin[4] = {fi, fi, fi, fi};  // stage i reads fin[i]; k_fused reads fin[n_stages]
     if (dense) {
-        const int most = std::max({n_groups, n_small, n_large});
-        HIPCHK_RET(B->dscratch.ensure((size_t)most * D_SCRATCH));  // the stages run one at a time
         HIPCHK_RET(B->dense_ids.ensure((size_t)NP * 4));
         HIPCHK_RET(B->dense_ids2.ensure((size_t)NP * 4));
         if (n_stages > 2) HIPCHK_RET(B->dense_ids3.ensure((size_t)NP * 4));
-        dscratch = B->dscratch.as<uint8_t>();
         lists[0] = B->dense_ids.as<uint32_t>();
         lists[1] = B->dense_ids2.as<uint32_t>();
         lists[2] = B->dense_ids3.as<uint32_t>();
@@ -911,19 +907,19 @@ int run_batched(Context* ctx, const SegInput& in, OutputDesc& out, hipStream_t s
             const FusedIn& f = fin[i];
             if (stages[i] == D_SMALL)
                 hipLaunchKernelGGL((k_dense<2, 32>), dim3(n_small), dim3(2 * WAVE), 0, st, f, fo, ctl, queues[i],
-                                   lists[i], counts[i], dscratch);
+                                   lists[i], counts[i]);
             else if (stages[i] == D_LARGE)
                 hipLaunchKernelGGL((k_dense<16, 256>), dim3(n_large), dim3(16 * WAVE), 0, st, f, fo, ctl, queues[i],
-                                   lists[i], counts[i], dscratch);
+                                   lists[i], counts[i]);
             else if (dnw == 2)
                 hipLaunchKernelGGL((k_dense<2, 64>), dim3(n_groups), dim3(2 * WAVE), 0, st, f, fo, ctl, queues[i],
-                                   lists[i], counts[i], dscratch);
+                                   lists[i], counts[i]);
             else if (dnw == 8)
                 hipLaunchKernelGGL((k_dense<8, 64>), dim3(n_groups), dim3(8 * WAVE), 0, st, f, fo, ctl, queues[i],
-                                   lists[i], counts[i], dscratch);
+                                   lists[i], counts[i]);
             else
                 hipLaunchKernelGGL((k_dense<4, 64>), dim3(n_groups), dim3(4 * WAVE), 0, st, f, fo, ctl, queues[i],
-                                   lists[i], counts[i], dscratch);
+                                   lists[i], counts[i]);
         }
         if (dense && ev) (void)hipEventRecord(ctx->ev[2], st);
         const FusedIn& f = fin[n_stages];
@@ -939,7 +935,6 @@ int run_batched(Context* ctx, const SegInput& in, OutputDesc& out, hipStream_t s
         memcpy(key.data() + sizeof(fi) + sizeof(fo), &n_waves, 4);
         memcpy(key.data() + sizeof(fi) + sizeof(fo) + 4, &g, 4);
         memcpy(key.data() + sizeof(fi) + sizeof(fo) + 8, &scratch, 8);
-        memcpy(key.data() + sizeof(fi) + sizeof(fo) + 16, &dscratch, 8);
         const int dkey = dense ? (dnw << 16) | n_groups : 0;
         memcpy(key.data() + sizeof(fi) + sizeof(fo) + 24, &dkey, 4);
         memcpy(key.data() + sizeof(fi) + sizeof(fo) + 28, &n_small, 4);
@@ -1149,6 +1144,71 @@ int run_adjoint(Context* ctx, const SegInput& in, const AdjTarget& tgt, float* l
     if (hc->error & ERR_FLATTEN) return -5;
     if (hc->error & ERR_BBOX) return -3;
     if (hc->error & ERR_CAPACITY) return -7;
+    return 0;
+}
+
+// ------------------------------------------------ the dense chain alone
+//
+// The dense 64-tile instance over every path, then the 256-tile instance over
+// the paths it hands on, with control memory (DENSE_CTL_BYTES: queues,
+// counters, error bits) and the handed-on list (n_paths entries) supplied by
+// the caller: nothing is allocated.  Paths the 256-tile instance hands on are
+// counted, not listed.
+static_assert(sizeof(FusedCtl) == DENSE_CTL_BYTES, "control block size");
+
+int launch_dense_chain(const SegInput& in, const OutputDesc& out, uint8_t* ctl_mem, uint32_t* ids, hipStream_t s) {
+    const int64_t NP = in.n_paths;
+    if (NP <= 0) return 0;
+    int dev = 0, cus = 0;
+    HIPCHK_RET(hipGetDevice(&dev));
+    HIPCHK_RET(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    FusedCtl* ctl = reinterpret_cast<FusedCtl*>(ctl_mem);
+    FusedIn fi;
+    fi.n_paths = NP;
+    fi.verbs = in.verbs;
+    fi.verb_off = in.seg_verb_off;
+    fi.points = in.points;
+    fi.point_off = in.seg_point_off;
+    fi.weights = in.weights;
+    fi.weight_off = in.seg_weight_off;
+    fi.xf = in.seg_xf;
+    fi.n_points = in.n_points;
+    fi.n_weights = in.n_weights;
+    FusedOut fo;
+    fo.tile_xy = out.tile_xy;
+    fo.tile_alpha = out.tile_alpha;
+    fo.tile_cov = out.tile_cov;
+    fo.tile_cap = out.tile_capacity;
+    fo.span_xy = out.span_xy;
+    fo.span_w = out.span_w;
+    fo.span_cap = out.span_capacity;
+    fo.pts = out.path_tile_start;
+    fo.ptc = out.path_tile_count;
+    fo.pss = out.path_span_start;
+    fo.psc = out.path_span_count;
+    FusedIn f2 = fi;  // the 256-tile instance runs over the 64-tile one's hand-ons
+    f2.ids = ids;
+    f2.n_ids = &ctl->dq;
+    hipLaunchKernelGGL(k_fused_reset, dim3(1), dim3(64), 0, s, ctl);
+    hipLaunchKernelGGL((k_dense<4, 64>), dim3(cus * 4), dim3(4 * WAVE), 0, s, fi, fo, ctl, &ctl->queue, ids,
+                       &ctl->dq);
+    hipLaunchKernelGGL((k_dense<16, 256>), dim3(cus), dim3(16 * WAVE), 0, s, f2, fo, ctl, &ctl->queue3,
+                       (uint32_t*)nullptr, &ctl->dq2);
+    HIPCHK_RET(hipGetLastError());
+    return 0;
+}
+
+int finish_dense_chain(const uint8_t* ctl_mem, OutputDesc& out, hipStream_t s) {
+    FusedCtl hc;
+    HIPCHK_RET(hipMemcpyAsync(&hc, ctl_mem, sizeof(FusedCtl), hipMemcpyDeviceToHost, s));
+    HIPCHK_RET(hipStreamSynchronize(s));
+    if (hc.error & ERR_INPUT) return -1;
+    if (hc.error & ERR_FLATTEN) return -5;
+    if (hc.error & ERR_BBOX) return -3;
+    out.n_tiles = (int64_t)hc.tiles;
+    out.n_spans = (int64_t)hc.spans;
+    if ((hc.error & ERR_CAPACITY) || out.n_tiles > out.tile_capacity || out.n_spans > out.span_capacity) return -4;
+    if (hc.dq2) return -7;
     return 0;
 }
 

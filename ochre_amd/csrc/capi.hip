@@ -187,7 +187,7 @@ int ochre_rasterize_batched(ochre_context* ctx, const ochre_path_batch* in, ochr
 }
 
 static int stroke_batch(ochre_context* ctx, const ochre_path_batch* in, const float* widths,
-                        ochre_batch_output* out, void* stream, bool batched) {
+                        const uint8_t* stroke_flags, ochre_batch_output* out, void* stream, bool batched) {
     if (!ctx || !in || !out || (in->n_paths > 0 && !widths)) return OCHRE_ERR_INVALID_ARG;
     SegInput si;
     int st = to_seg_input(in, &si);
@@ -195,7 +195,7 @@ static int stroke_batch(ochre_context* ctx, const ochre_path_batch* in, const fl
     std::lock_guard<std::mutex> lk(ctx->mu);
     if (hipSetDevice(ctx->c.device) != hipSuccess) return OCHRE_ERR_DEVICE;
     OutputDesc od = to_output(out);
-    st = run_stroke(&ctx->c, si, widths, od, reinterpret_cast<hipStream_t>(stream), batched);
+    st = run_stroke(&ctx->c, si, widths, stroke_flags, od, reinterpret_cast<hipStream_t>(stream), batched);
     out->n_tiles = od.n_tiles;
     out->n_spans = od.n_spans;
     return st;
@@ -203,12 +203,24 @@ static int stroke_batch(ochre_context* ctx, const ochre_path_batch* in, const fl
 
 int ochre_stroke_plain(ochre_context* ctx, const ochre_path_batch* in, const float* widths, ochre_batch_output* out,
                        void* stream) {
-    return stroke_batch(ctx, in, widths, out, stream, false);
+    return stroke_batch(ctx, in, widths, nullptr, out, stream, false);
 }
 
 int ochre_stroke_batched(ochre_context* ctx, const ochre_path_batch* in, const float* widths,
                          ochre_batch_output* out, void* stream) {
-    return stroke_batch(ctx, in, widths, out, stream, true);
+    return stroke_batch(ctx, in, widths, nullptr, out, stream, true);
+}
+
+int ochre_draw_plain(ochre_context* ctx, const ochre_path_batch* in, const uint8_t* stroke_flags,
+                     const float* widths, ochre_batch_output* out, void* stream) {
+    if (in && in->n_paths > 0 && !stroke_flags) return OCHRE_ERR_INVALID_ARG;
+    return stroke_batch(ctx, in, widths, stroke_flags, out, stream, false);
+}
+
+int ochre_draw_batched(ochre_context* ctx, const ochre_path_batch* in, const uint8_t* stroke_flags,
+                       const float* widths, ochre_batch_output* out, void* stream) {
+    if (in && in->n_paths > 0 && !stroke_flags) return OCHRE_ERR_INVALID_ARG;
+    return stroke_batch(ctx, in, widths, stroke_flags, out, stream, true);
 }
 
 int ochre_build_atlas(ochre_context* ctx, const ochre_batch_output* in, int64_t n_paths, const uint8_t* colors,
@@ -590,7 +602,7 @@ int expand_strokes(ochre_context* ctx, ochre_rasterizer* r, DevPath* dp, hipStre
     si.n_points = (int64_t)r->s_points.size() / 2;
     si.n_weights = (int64_t)r->s_weights.size();
     StrokeOutline o;
-    if ((st = stroke_expand(&ctx->c, si, dp->s_width.as<float>(), o, s))) return st;
+    if ((st = stroke_expand(&ctx->c, si, dp->s_width.as<float>(), nullptr, o, s))) return st;
     std::vector<int64_t> ov((size_t)K + 1), op((size_t)K + 1);
     std::vector<uint8_t> overbs((size_t)o.in.n_verbs);
     std::vector<float> opts((size_t)o.in.n_points * 2);

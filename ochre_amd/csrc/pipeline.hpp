@@ -93,7 +93,7 @@ struct DBuf {
 
 // Device buffers of the stroke expansion (flattened polylines, outlines).
 struct StrokeBuffers {
-    DBuf cnt, fv, fp, ov, op, ow, err, fverbs, fpoints, overbs, opoints;
+    DBuf cnt, fv, fp, ov, op, ow, err, fverbs, fpoints, overbs, opoints, oweights;
     void* host = nullptr;  // pinned: sizes and error bits read back
     ~StrokeBuffers();
 };
@@ -167,6 +167,15 @@ int run_plain(Context* ctx, const SegInput& in, OutputDesc& out, hipStream_t str
 // a captured HIP graph.
 int run_batched(Context* ctx, const SegInput& in, OutputDesc& out, hipStream_t stream, int waves_per_cu, int gran,
                 int tiles_per_chunk, bool use_graph);
+// The dense kernels alone (the 64-tile instance, then the 256-tile one over
+// what it hands on) with caller-provided device memory: `ctl` of
+// DENSE_CTL_BYTES (cleared by a kernel at the start) and `ids` of n_paths
+// entries.  finish_dense_chain reads the control block back after the launch:
+// the statuses of run_batched, and -7 when a path was over the 256-tile
+// instance (its output range is not written).  Neither allocates.
+constexpr size_t DENSE_CTL_BYTES = 256;
+int launch_dense_chain(const SegInput& in, const OutputDesc& out, uint8_t* ctl, uint32_t* ids, hipStream_t stream);
+int finish_dense_chain(const uint8_t* ctl, OutputDesc& out, hipStream_t stream);
 
 // Target render of the coverage adjoint (device pointers, per path ranges).
 struct AdjTarget {
@@ -189,11 +198,14 @@ struct StrokeOutline {
     SegInput in;
 };
 // Rasterizer::stroke for every path of a batch: path-space flattening and
-// stroke expansion on the device (widths: [n_paths], device memory).
-int stroke_expand(Context* ctx, const SegInput& in, const float* widths, StrokeOutline& out, hipStream_t stream);
+// stroke expansion on the device (widths: [n_paths], device memory).  With
+// stroke_flags (device, [n_paths]) the batch is a draw list: a path whose
+// flag is 0 is a fill and passes through unchanged.
+int stroke_expand(Context* ctx, const SegInput& in, const float* widths, const uint8_t* stroke_flags,
+                  StrokeOutline& out, hipStream_t stream);
 // stroke_expand, then the plain or batched fill of the outlines.
-int run_stroke(Context* ctx, const SegInput& in, const float* widths, OutputDesc& out, hipStream_t stream,
-               bool batched);
+int run_stroke(Context* ctx, const SegInput& in, const float* widths, const uint8_t* stroke_flags, OutputDesc& out,
+               hipStream_t stream, bool batched);
 
 }  // namespace ochre
 struct ochre_atlas_vertex;

@@ -41,12 +41,16 @@ struct PathSrc {
 // command's end point, which is the point before the command's own.
 // Returns the error bits of the lane.
 template <bool EMIT>
-__global__ __launch_bounds__(WAVE) void k_sflatten(PathSrc in, int64_t n, int64_t* cnt, const int64_t* off_v,
-                                                  const int64_t* off_p, uint8_t* verbs, float* points,
-                                                  uint32_t* error) {
+__global__ __launch_bounds__(WAVE) void k_sflatten(PathSrc in, const uint8_t* stroke_flags, int64_t n, int64_t* cnt,
+                                                  const int64_t* off_v, const int64_t* off_p, uint8_t* verbs,
+                                                  float* points, uint32_t* error) {
     const int64_t p = blockIdx.x;
     if (p >= n) return;
     const int lane = threadIdx.x;
+    if (stroke_flags && !stroke_flags[p]) {  // a fill of a draw list: not flattened here
+        if (!EMIT && lane == 0) cnt[p] = cnt[n + p] = 0;
+        return;
+    }
     const int64_t vb = in.verb_off[p], ve = in.verb_off[p + 1];
     const int64_t pb = in.point_off[p], pe = in.point_off[p + 1];
     const int64_t wb = in.weight_off[p], we = in.weight_off[p + 1];
@@ -235,10 +239,29 @@ template <bool EMIT>
 __global__ __launch_bounds__(WAVE) void k_sstroke(const uint8_t* fverbs, const float* fpoints, const int64_t* foff_v,
                                                  const int64_t* foff_p, const float* widths, int64_t n, int64_t* cnt,
                                                  const int64_t* off_v, const int64_t* off_p, uint8_t* verbs,
-                                                 float* points) {
+                                                 float* points, const uint8_t* stroke_flags, PathSrc src,
+                                                 const int64_t* off_w, float* weights) {
     const int64_t p = blockIdx.x;
     if (p >= n) return;
     const int lane = threadIdx.x;
+    if (stroke_flags && !stroke_flags[p]) {
+        // a fill of a draw list: its commands pass through to the fill unchanged
+        const int64_t vb = src.verb_off[p], nv = src.verb_off[p + 1] - vb;
+        const int64_t pb = src.point_off[p], np = src.point_off[p + 1] - pb;
+        const int64_t wb = src.weight_off[p], nw = src.weight_off[p + 1] - wb;
+        if (!EMIT) {
+            if (lane == 0) {
+                cnt[p] = nv;
+                cnt[n + p] = np;
+                cnt[2 * n + p] = nw;
+            }
+            return;
+        }
+        for (int64_t k = lane; k < nv; k += WAVE) verbs[off_v[p] + k] = src.verbs[vb + k];
+        for (int64_t k = lane; k < 2 * np; k += WAVE) points[2 * off_p[p] + k] = src.points[2 * pb + k];
+        for (int64_t k = lane; k < nw; k += WAVE) weights[off_w[p] + k] = src.weights[wb + k];
+        return;
+    }
     const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     const uint8_t* v = fverbs + foff_v[p];
     const float* pts = fpoints + 2 * foff_p[p];
@@ -282,6 +305,7 @@ __global__ __launch_bounds__(WAVE) void k_sstroke(const uint8_t* fverbs, const f
     if (!EMIT && lane == 0) {
         cnt[p] = (int64_t)o.v;
         cnt[n + p] = (int64_t)o.q;
+        if (stroke_flags) cnt[2 * n + p] = 0;  // outlines hold no conics
     }
 }
 
@@ -321,13 +345,14 @@ hipError_t offsets_i64(Context* ctx, const int64_t* cnt, int64_t* off, int64_t n
 
 }  // namespace
 
-int stroke_expand(Context* ctx, const SegInput& in, const float* widths, StrokeOutline& o, hipStream_t s) {
+int stroke_expand(Context* ctx, const SegInput& in, const float* widths, const uint8_t* stroke_flags,
+                  StrokeOutline& o, hipStream_t s) {
     const int64_t n = in.n_paths;
     o = StrokeOutline{};
     if (n <= 0) return 0;
     if (in.n_segs != n || in.seg_raw || in.seg_path || in.path_seg_off || !widths) return -1;
     StrokeBuffers& B = ctx->stroke;
-    HIPCHK_RET(B.cnt.ensure((size_t)n * 16));
+    HIPCHK_RET(B.cnt.ensure((size_t)n * 24));
     HIPCHK_RET(B.fv.ensure((size_t)(n + 1) * 8));
     HIPCHK_RET(B.fp.ensure((size_t)(n + 1) * 8));
     HIPCHK_RET(B.ov.ensure((size_t)(n + 1) * 8));
@@ -341,8 +366,8 @@ int stroke_expand(Context* ctx, const SegInput& in, const float* widths, StrokeO
     uint32_t* err = B.err.as<uint32_t>();
     const uint32_t blocks = (uint32_t)n;  // one wave per path
     hipLaunchKernelGGL(k_zero_i64, dim3(1), dim3(64), 0, s, B.err.as<int64_t>(), (int64_t)2);
-    hipLaunchKernelGGL(k_sflatten<false>, dim3(blocks), dim3(WAVE), 0, s, src, n, cnt, nullptr, nullptr, nullptr,
-                       nullptr, err);
+    hipLaunchKernelGGL(k_sflatten<false>, dim3(blocks), dim3(WAVE), 0, s, src, stroke_flags, n, cnt, nullptr,
+                       nullptr, nullptr, nullptr, err);
     HIPCHK_RET(offsets_i64(ctx, cnt, B.fv.as<int64_t>(), n, s));
     HIPCHK_RET(offsets_i64(ctx, cnt + n, B.fp.as<int64_t>(), n, s));
     HIPCHK_RET(hipMemcpyAsync(h, B.fv.as<int64_t>() + n, 8, hipMemcpyDeviceToHost, s));
@@ -354,45 +379,51 @@ int stroke_expand(Context* ctx, const SegInput& in, const float* widths, StrokeO
     const int64_t FV = h[0], FP = h[1];
     HIPCHK_RET(B.fverbs.ensure((size_t)std::max<int64_t>(FV, 1)));
     HIPCHK_RET(B.fpoints.ensure((size_t)std::max<int64_t>(FP, 1) * 8));
-    hipLaunchKernelGGL(k_sflatten<true>, dim3(blocks), dim3(WAVE), 0, s, src, n, cnt, B.fv.as<int64_t>(),
-                       B.fp.as<int64_t>(), B.fverbs.as<uint8_t>(), B.fpoints.as<float>(), err);
+    hipLaunchKernelGGL(k_sflatten<true>, dim3(blocks), dim3(WAVE), 0, s, src, stroke_flags, n, cnt,
+                       B.fv.as<int64_t>(), B.fp.as<int64_t>(), B.fverbs.as<uint8_t>(), B.fpoints.as<float>(), err);
     hipLaunchKernelGGL(k_sstroke<false>, dim3(blocks), dim3(WAVE), 0, s, B.fverbs.as<uint8_t>(),
                        B.fpoints.as<float>(), B.fv.as<int64_t>(), B.fp.as<int64_t>(), widths, n, cnt, nullptr,
-                       nullptr, nullptr, nullptr);
+                       nullptr, nullptr, nullptr, stroke_flags, src, nullptr, nullptr);
     HIPCHK_RET(offsets_i64(ctx, cnt, B.ov.as<int64_t>(), n, s));
     HIPCHK_RET(offsets_i64(ctx, cnt + n, B.op.as<int64_t>(), n, s));
+    if (stroke_flags) HIPCHK_RET(offsets_i64(ctx, cnt + 2 * n, B.ow.as<int64_t>(), n, s));
     HIPCHK_RET(hipMemcpyAsync(h, B.ov.as<int64_t>() + n, 8, hipMemcpyDeviceToHost, s));
     HIPCHK_RET(hipMemcpyAsync(h + 1, B.op.as<int64_t>() + n, 8, hipMemcpyDeviceToHost, s));
+    if (stroke_flags) HIPCHK_RET(hipMemcpyAsync(h + 2, B.ow.as<int64_t>() + n, 8, hipMemcpyDeviceToHost, s));
     HIPCHK_RET(hipStreamSynchronize(s));
-    const int64_t OV = h[0], OP = h[1];
+    const int64_t OV = h[0], OP = h[1], OW = stroke_flags ? h[2] : 0;
     HIPCHK_RET(B.overbs.ensure((size_t)std::max<int64_t>(OV, 1)));
     HIPCHK_RET(B.opoints.ensure((size_t)std::max<int64_t>(OP, 1) * 8));
+    HIPCHK_RET(B.oweights.ensure((size_t)std::max<int64_t>(OW, 1) * 4));
     hipLaunchKernelGGL(k_sstroke<true>, dim3(blocks), dim3(WAVE), 0, s, B.fverbs.as<uint8_t>(),
                        B.fpoints.as<float>(), B.fv.as<int64_t>(), B.fp.as<int64_t>(), widths, n, cnt,
-                       B.ov.as<int64_t>(), B.op.as<int64_t>(), B.overbs.as<uint8_t>(), B.opoints.as<float>());
-    // outlines hold no conics: every weight offset is zero
-    hipLaunchKernelGGL(k_zero_i64, dim3((uint32_t)std::min<int64_t>((n + 256) / 256, 1024)), dim3(256), 0, s,
-                       B.ow.as<int64_t>(), n + 1);
+                       B.ov.as<int64_t>(), B.op.as<int64_t>(), B.overbs.as<uint8_t>(), B.opoints.as<float>(),
+                       stroke_flags, src, B.ow.as<int64_t>(), B.oweights.as<float>());
+    // stroke outlines hold no conics: with no fills every weight offset is zero
+    if (!stroke_flags)
+        hipLaunchKernelGGL(k_zero_i64, dim3((uint32_t)std::min<int64_t>((n + 256) / 256, 1024)), dim3(256), 0, s,
+                           B.ow.as<int64_t>(), n + 1);
     HIPCHK_RET(hipGetLastError());
     o.in.n_segs = n;
     o.in.n_paths = n;
     o.in.verbs = B.overbs.as<uint8_t>();
     o.in.points = B.opoints.as<float>();
-    o.in.weights = nullptr;
+    o.in.weights = OW ? B.oweights.as<float>() : nullptr;
     o.in.seg_verb_off = B.ov.as<int64_t>();
     o.in.seg_point_off = B.op.as<int64_t>();
     o.in.seg_weight_off = B.ow.as<int64_t>();
     o.in.seg_xf = in.seg_xf;
     o.in.n_verbs = OV;
     o.in.n_points = OP;
-    o.in.n_weights = 0;
+    o.in.n_weights = OW;
     return 0;
 }
 
-int run_stroke(Context* ctx, const SegInput& in, const float* widths, OutputDesc& out, hipStream_t s, bool batched) {
+int run_stroke(Context* ctx, const SegInput& in, const float* widths, const uint8_t* stroke_flags, OutputDesc& out,
+               hipStream_t s, bool batched) {
     out.n_tiles = out.n_spans = 0;
     StrokeOutline o;
-    int st = stroke_expand(ctx, in, widths, o, s);
+    int st = stroke_expand(ctx, in, widths, stroke_flags, o, s);
     if (st || in.n_paths <= 0) return st;
     if (batched)
         return run_batched(ctx, o.in, out, s, ctx->batched_waves_per_cu, ctx->batched_gran, ctx->batched_tc,

@@ -200,6 +200,21 @@ def stroke(ctx: Context, db: DeviceBatch, widths, out: DeviceOutput, batched=Tru
     return co.n_tiles, co.n_spans
 
 
+def draw(ctx: Context, db: DeviceBatch, stroke_flags, widths, out: DeviceOutput, batched=True, stream=None):
+    """A draw list: path p filled (stroke_flags[p] == 0) or stroked with
+    widths[p] (stroke_flags[p] == 1), each its own Rasterizer (device uint8
+    and float32 tensors of n_paths).  Returns (n_tiles, n_spans)."""
+    cb = db.c()
+    co = out.c()
+    fn = ctx.lib.ochre_draw_batched if batched else ctx.lib.ochre_draw_plain
+    fp = stroke_flags.data_ptr() if stroke_flags.numel() else None
+    wp = widths.data_ptr() if widths.numel() else None
+    st = fn(ctx.h, ctypes.byref(cb), fp, wp, ctypes.byref(co), _stream_ptr(stream))
+    out.n_tiles, out.n_spans = co.n_tiles, co.n_spans
+    _lib.check(st, "ochre_draw_batched" if batched else "ochre_draw_plain")
+    return co.n_tiles, co.n_spans
+
+
 ATLAS_VERTEX_BYTES = 12  # i16 pos[2], u16 uv[2], u8 col[4] (examples/svg.rs:15-20)
 
 
@@ -230,11 +245,12 @@ def run_to_host(ctx, batch, mode="plain", tile_capacity=None, span_capacity=None
     and return the per-path host arrays.  Modes: "plain", "batched" (the
     ordered batched kernel alone), "dense" (the dense kernels in front of it),
     and the same for strokes (batch.widths): "stroke_plain",
-    "stroke_batched", "stroke_dense".  The context's dense setting is
-    restored afterwards."""
+    "stroke_batched", "stroke_dense", and for draw lists (batch.strokes,
+    batch.widths): "draw_plain", "draw_batched", "draw_dense".  The
+    context's dense setting is restored afterwards."""
     d = dense_params(ctx)
-    if mode in ("batched", "dense", "stroke_batched", "stroke_dense"):
-        set_dense_params(ctx, mode in ("dense", "stroke_dense"), d["dense_groups_per_cu"],
+    if mode in ("batched", "dense", "stroke_batched", "stroke_dense", "draw_batched", "draw_dense"):
+        set_dense_params(ctx, mode in ("dense", "stroke_dense", "draw_dense"), d["dense_groups_per_cu"],
                          d["dense_waves_per_group"])
     try:
         return _run_to_host(ctx, batch, mode, tile_capacity, span_capacity)
@@ -250,6 +266,10 @@ def _run_to_host(ctx, batch, mode, tile_capacity, span_capacity):
     if mode.startswith("stroke"):
         wd = torch.from_numpy(np.ascontiguousarray(batch.widths, np.float32)).to("cuda")
         fn = lambda c, d, o: stroke(c, d, wd, o, batched=mode != "stroke_plain")  # noqa: E731
+    elif mode.startswith("draw"):
+        wd = torch.from_numpy(np.ascontiguousarray(batch.widths, np.float32)).to("cuda")
+        fl = torch.from_numpy(np.ascontiguousarray(batch.strokes, np.uint8)).to("cuda")
+        fn = lambda c, d, o: draw(c, d, fl, wd, o, batched=mode != "draw_plain")  # noqa: E731
     else:
         fn = rasterize_plain if mode == "plain" else rasterize_batched
     for _ in range(2):

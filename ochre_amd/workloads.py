@@ -44,6 +44,8 @@ class PathBatch:
     xforms: np.ndarray       # f32 [n, 6]: m00 m01 m10 m11 ox oy (row-major Mat2x2 + offset)
     widths: np.ndarray | None = None   # f32 [n] stroke widths (stroke mode)
     scene: np.ndarray | None = None    # i32 [n] scene id (batched scenes)
+    strokes: np.ndarray | None = None  # u8 [n] draw list: 1 stroke (with widths), 0 fill
+    colors: np.ndarray | None = None   # u8 [n, 4] draw colours (SVG draw lists)
     name: str = ""
 
     @property
@@ -66,6 +68,10 @@ class PathBatch:
             b.widths = self.widths[idx].copy()
         if self.scene is not None:
             b.scene = self.scene[idx].copy()
+        if self.strokes is not None:
+            b.strokes = self.strokes[idx].copy()
+        if self.colors is not None:
+            b.colors = self.colors[idx].copy()
         b.name = self.name + "[subset]"
         return b
 

@@ -710,7 +710,7 @@ orc_result* orc_run_batch(int64_t n_paths,
                           const uint8_t* verbs, const int64_t* verb_off,
                           const float* points, const int64_t* point_off,
                           const float* weights, const int64_t* weight_off,
-                          const float* xforms, const float* widths) {
+                          const float* xforms, const float* widths, const uint8_t* stroke_flags) {
     orc_result* res = (orc_result*)calloc(1, sizeof(orc_result));
     if (!res) abort();
     res->n_paths = n_paths;
@@ -725,7 +725,8 @@ orc_result* orc_run_batch(int64_t n_paths,
         const float* w = weights ? weights + (weight_off ? weight_off[p] : 0) : NULL;
         size_t nw = weight_off ? (size_t)(weight_off[p + 1] - weight_off[p]) : 0;
         int st;
-        if (widths) st = orc_stroke(r, v, nv, pts, np, w, nw, widths[p], xforms + 6 * p);
+        if (widths && (!stroke_flags || stroke_flags[p]))
+            st = orc_stroke(r, v, nv, pts, np, w, nw, widths[p], xforms + 6 * p);
         else st = orc_fill(r, v, nv, pts, np, w, nw, xforms + 6 * p);
         res->status |= st;
         st = orc_finish(r, res, res_tile, res_span);

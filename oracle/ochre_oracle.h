@@ -110,17 +110,21 @@ void orc_path_free(orc_path* p);
  * report the required counts). */
 int orc_atlas_batch(int64_t n_paths, const uint8_t* verbs, const int64_t* verb_off, const float* points,
                     const int64_t* point_off, const float* weights, const int64_t* weight_off, const float* xforms,
-                    const float* widths, const uint8_t* colors, int atlas_size, uint8_t* atlas, void* verts,
-                    size_t vcap, uint32_t* idx, size_t icap, size_t* n_verts, size_t* n_idx);
+                    const float* widths, const uint8_t* stroke_flags, const uint8_t* colors, int atlas_size,
+                    uint8_t* atlas, void* verts, size_t vcap, uint32_t* idx, size_t icap, size_t* n_verts,
+                    size_t* n_idx);
 
 /* Batch runner: one Rasterizer per path (as examples/svg.rs:143-145 does),
- * fill (or stroke when widths != NULL) with the path's transform, finish. */
+ * fill (or stroke when widths != NULL) with the path's transform, finish.
+ * With widths and stroke_flags both set the batch is a draw list: path p is
+ * stroked when stroke_flags[p] != 0 and filled otherwise (the demo's
+ * sequence of fills and strokes, svg.rs:140-156). */
 typedef struct orc_result orc_result;
 orc_result* orc_run_batch(int64_t n_paths,
                           const uint8_t* verbs, const int64_t* verb_off,
                           const float* points, const int64_t* point_off,
                           const float* weights, const int64_t* weight_off,
-                          const float* xforms, const float* widths);
+                          const float* xforms, const float* widths, const uint8_t* stroke_flags);
 int orc_result_status(const orc_result* res);
 void orc_result_sizes(const orc_result* res, int64_t* n_tiles, int64_t* n_spans);
 void orc_result_copy(const orc_result* res, int16_t* tile_xy, uint8_t* tile_alpha,

@@ -356,6 +356,7 @@ typedef struct {
     int k0, k1;
     double b0, b1;
     double area, height;
+    int virt; /* a zero-length derivative carrier (see walk): no tile, no sums */
 } incr;
 
 typedef struct {
@@ -411,6 +412,21 @@ static void walk(d2 last, d2 point, incrs* I, tincs* T, int16_t* ty_prev) {
             I->v = (incr*)realloc(I->v, I->cap * sizeof(incr));
             if (!I->v) abort();
         }
+        /* A line ending exactly on the next pixel boundary in its direction
+         * of travel stops in the pixel before it, while the next line starts
+         * beyond it (the floor of its start point).  Both lines take their
+         * derivatives on the same side of that tie: this increment ends at
+         * the boundary, and a zero-length increment in the pixel beyond
+         * carries the end point's derivative.  (A corner tie keeps the end.) */
+        int vx = 0, vy = 0;
+        if (k1 == 0) {
+            int on_y = yd > 0 && point.y == (double)(int16_t)(y + 1);
+            int on_x = xd > 0 && point.x == (double)(int16_t)(x + 1);
+            if (on_y != on_x) {
+                if (on_y) { k1 = 1; b1 = (double)(int16_t)(y + 1); vy = 1; }
+                else { k1 = 2; b1 = (double)(int16_t)(x + 1); vx = 1; }
+            }
+        }
         incr* in = &I->v[I->n++];
         in->x = x;
         in->y = y;
@@ -422,6 +438,26 @@ static void walk(d2 last, d2 point, incrs* I, tincs* T, int16_t* ty_prev) {
         in->b1 = b1;
         in->area = area;
         in->height = h;
+        in->virt = 0;
+        if (vx || vy) {
+            if (I->n == I->cap) {
+                I->cap = 2 * I->cap;
+                I->v = (incr*)realloc(I->v, I->cap * sizeof(incr));
+                if (!I->v) abort();
+            }
+            incr* vi = &I->v[I->n++];
+            vi->x = (int16_t)(x + vx);
+            vi->y = (int16_t)(y + vy);
+            vi->t0 = t1;
+            vi->t1 = t1;
+            vi->k0 = k1;
+            vi->b0 = b1;
+            vi->k1 = 0;
+            vi->b1 = 0.0;
+            vi->area = 0.0;
+            vi->height = 0.0;
+            vi->virt = 1;
+        }
         if (rt1 < ct1) {
             rt0 = rt1;
             rt1 = fmin(rt1 + ys, 1.0);
@@ -510,6 +546,7 @@ static void build_render(const incrs* I, tincs* T, render* R) {
     size_t cap = 64;
     R->t = (tile*)malloc(cap * sizeof(tile));
     for (size_t i = 0; i < I->n; i++) {
+        if (I->v[i].virt) continue;
         int16_t tx = div8(I->v[i].x), ty = div8(I->v[i].y);
         int seen = 0;
         for (size_t k = R->n; k-- > 0 && k + 64 >= R->n;)
@@ -534,6 +571,7 @@ static void build_render(const incrs* I, tincs* T, render* R) {
     }
     qsort(R->t, R->n, sizeof(tile), tile_cmp);
     for (size_t i = 0; i < I->n; i++) {
+        if (I->v[i].virt) continue;
         long k = find_tile(R, div8(I->v[i].x), div8(I->v[i].y));
         int p = (I->v[i].y & 7) * 8 + (I->v[i].x & 7);
         R->t[k].a[p] += I->v[i].area;
@@ -798,6 +836,7 @@ int orc64_path_loss(const uint8_t* verbs, size_t nv, const double* points, size_
             const incr* in = &I.v[i];
             const line* ln = &L.v[line_of[i]];
             long k = find_tile(&R, div8(in->x), div8(in->y));
+            if (k < 0) continue; /* a zero-length carrier outside the path's tiles */
             int p = (in->y & 7) * 8 + (in->x & 7);
             double gA = R.t[k].g[p], gH = R.t[k].G[p];
             if (gA == 0.0 && gH == 0.0) continue;

@@ -78,7 +78,8 @@ static void on_span(void* user, int16_t x, int16_t y, uint16_t width) {
 
 int orc_atlas_batch(int64_t n_paths, const uint8_t* verbs, const int64_t* verb_off, const float* points,
                     const int64_t* point_off, const float* weights, const int64_t* weight_off, const float* xforms,
-                    const float* widths, const uint8_t* colors, int atlas_size, uint8_t* atlas, void* verts,
+                    const float* widths, const uint8_t* stroke_flags, const uint8_t* colors, int atlas_size,
+                    uint8_t* atlas, void* verts,
                     size_t vcap, uint32_t* idx, size_t icap, size_t* n_verts, size_t* n_idx) {
     builder b;
     memset(&b, 0, sizeof(b));
@@ -103,7 +104,8 @@ int orc_atlas_batch(int64_t n_paths, const uint8_t* verbs, const int64_t* verb_o
         size_t np = (size_t)(point_off[p + 1] - point_off[p]);
         const float* w = weights ? weights + weight_off[p] : NULL;
         size_t nw = (size_t)(weight_off[p + 1] - weight_off[p]);
-        if (widths) st |= orc_stroke(r, v, nv, pts, np, w, nw, widths[p], xforms + 6 * p);
+        if (widths && (!stroke_flags || stroke_flags[p]))
+            st |= orc_stroke(r, v, nv, pts, np, w, nw, widths[p], xforms + 6 * p);
         else st |= orc_fill(r, v, nv, pts, np, w, nw, xforms + 6 * p);
         st |= orc_finish(r, &b, on_tile, on_span);
         orc_free(r);

@@ -30,7 +30,7 @@ def load():
     P = ctypes.c_void_p
     i64 = ctypes.c_int64
     lib.orc_run_batch.restype = P
-    lib.orc_run_batch.argtypes = [i64, P, P, P, P, P, P, P, P]
+    lib.orc_run_batch.argtypes = [i64, P, P, P, P, P, P, P, P, P]
     lib.orc_result_status.argtypes = [P]
     lib.orc_result_sizes.argtypes = [P, P, P]
     lib.orc_result_copy.argtypes = [P] * 7
@@ -81,6 +81,18 @@ class RasterOutput:
         return int(self.span_xy.shape[0])
 
 
+def _stroke_args(batch, stroke):
+    """(widths, stroke_flags) for the oracle: stroke=True strokes every path
+    with batch.widths; a batch carrying `strokes` (a draw list) strokes the
+    paths flagged 1 and fills the others."""
+    flags = getattr(batch, "strokes", None)
+    if flags is not None:
+        return (np.ascontiguousarray(batch.widths, np.float32), np.ascontiguousarray(flags, np.uint8))
+    if stroke:
+        return np.ascontiguousarray(batch.widths, np.float32), None
+    return None, None
+
+
 def run_batch(batch, stroke=False) -> RasterOutput:
     lib = load()
     v = np.ascontiguousarray(batch.verbs, np.uint8)
@@ -90,9 +102,9 @@ def run_batch(batch, stroke=False) -> RasterOutput:
     w = np.ascontiguousarray(batch.weights, np.float32)
     wo = np.ascontiguousarray(batch.weight_off, np.int64)
     xf = np.ascontiguousarray(batch.xforms, np.float32)
-    widths = np.ascontiguousarray(batch.widths, np.float32) if stroke else None
+    widths, flags = _stroke_args(batch, stroke)
     res = lib.orc_run_batch(batch.n_paths, _ptr(v), _ptr(vo), _ptr(p), _ptr(po),
-                            _ptr(w), _ptr(wo), _ptr(xf), _ptr(widths))
+                            _ptr(w), _ptr(wo), _ptr(xf), _ptr(widths), _ptr(flags))
     try:
         nt = ctypes.c_int64()
         ns = ctypes.c_int64()
@@ -130,7 +142,7 @@ def atlas_batch(batch, colors, atlas_size=4096, stroke=False):
     fn = lib.orc_atlas_batch
     fn.restype = ctypes.c_int
     P = ctypes.c_void_p
-    fn.argtypes = [ctypes.c_int64, P, P, P, P, P, P, P, P, P, ctypes.c_int, P, P, ctypes.c_size_t, P,
+    fn.argtypes = [ctypes.c_int64, P, P, P, P, P, P, P, P, P, P, ctypes.c_int, P, P, ctypes.c_size_t, P,
                    ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]
     v = np.ascontiguousarray(batch.verbs, np.uint8)
     vo = np.ascontiguousarray(batch.verb_off, np.int64)
@@ -139,12 +151,13 @@ def atlas_batch(batch, colors, atlas_size=4096, stroke=False):
     w = np.ascontiguousarray(batch.weights, np.float32)
     wo = np.ascontiguousarray(batch.weight_off, np.int64)
     xf = np.ascontiguousarray(batch.xforms, np.float32)
-    widths = np.ascontiguousarray(batch.widths, np.float32) if stroke else None
+    widths, flags = _stroke_args(batch, stroke)
     col = np.ascontiguousarray(colors, np.uint8).reshape(-1, 4)
     atlas = np.empty((atlas_size, atlas_size), np.uint8)
     nv, ni = ctypes.c_size_t(), ctypes.c_size_t()
     args = [batch.n_paths, _ptr(v), _ptr(vo), _ptr(p), _ptr(po), _ptr(w), _ptr(wo), _ptr(xf),
-            _ptr(widths) if widths is not None else None, _ptr(col), atlas_size, _ptr(atlas)]
+            _ptr(widths) if widths is not None else None, _ptr(flags) if flags is not None else None, _ptr(col),
+            atlas_size, _ptr(atlas)]
     fn(*args, None, 0, None, 0, ctypes.byref(nv), ctypes.byref(ni))
     verts = np.zeros(nv.value, ATLAS_VERTEX)
     idx = np.zeros(ni.value, np.uint32)

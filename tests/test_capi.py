@@ -42,3 +42,32 @@ def test_context_create_without_gpu_fails_cleanly():
     h = ctypes.c_void_p()
     assert lib.ochre_context_create(0, ctypes.byref(h)) == -6
     assert lib.ochre_rasterizer_new() is None
+
+
+def declared_signatures():
+    """{name: [parameter declarations]} from include/ochre.h."""
+    src = open(os.path.join(ROOT, "include", "ochre.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    sigs = {}
+    for m in re.finditer(r"\b(ochre_[a-z_]+)\s*\(([^;{]*?)\)\s*;", src):
+        params = [p.strip() for p in m.group(2).split(",") if p.strip() and p.strip() != "void"]
+        sigs[m.group(1)] = params
+    return sigs
+
+
+def test_every_pointer_taking_entry_point_has_declared_argtypes():
+    """ctypes passes an undeclared Python int argument as a 32-bit C int: a
+    device pointer handed to an entry point without argtypes is truncated and
+    the kernel reading it faults.  Every function of the header whose
+    parameters include a pointer, a 64-bit integer, a size or a float must
+    have argtypes of the right length once the library is loaded."""
+    lib = _lib.load()
+    wide = re.compile(r"\*|int64_t|size_t|float|double")
+    missing = []
+    for name, params in declared_signatures().items():
+        if not any(wide.search(p) for p in params):
+            continue
+        at = getattr(lib, name).argtypes
+        if at is None or len(at) != len(params):
+            missing.append((name, len(params), None if at is None else len(at)))
+    assert not missing, missing

@@ -76,3 +76,30 @@ def test_adjoint_matches_finite_differences(seed):
             assert abs(dx[k] - c) <= 1e-3 * max(abs(c), 1.0) + 1e-6, (i, "xf", k, dx[k], c)
             checked += 1
     assert checked >= 20
+
+
+@pytest.mark.parametrize("axis", [0, 1])
+def test_adjoint_at_a_vertex_on_a_pixel_boundary(axis):
+    """A vertex lying exactly on a pixel boundary inside a monotone edge: the
+    coverage is smooth there (the edge crosses the boundary once whichever
+    side the vertex is on), so the adjoint must match central differences
+    straddling the tie.  The walk of the line ending on the boundary stops in
+    the pixel before it while the next line starts beyond it; the two lines'
+    derivatives have to be taken on the same side of the tie."""
+    if axis == 1:  # B on the row boundary y = 12, both edges going down
+        pts = [(3.3, 2.6), (7.45, 12.0), (9.8, 21.3), (20.2, 19.7), (18.1, 4.4)]
+    else:  # B on the column boundary x = 12, both edges going right
+        pts = [(2.6, 3.3), (12.0, 7.45), (21.3, 9.8), (19.7, 20.2), (4.4, 18.1)]
+    v = np.array([0, 1, 1, 1, 1, 5], np.uint8)
+    p = np.array(pts, np.float64)
+    xf = np.array([1, 0, 0, 1, 0, 0], np.float64)
+    pt = p + np.array([[0.4, -0.3], [0.2, 0.5], [-0.3, 0.2], [0.1, -0.4], [-0.2, 0.3]])
+    st, _, _, _, (txy, cov) = O.path_loss64(v, pt, np.zeros(0), xf, want_cov=True)
+    tgt = O.Target(txy, cov.astype(np.float32), np.zeros((0, 2)), np.zeros(0))
+    st, L, dp, dx, _ = O.path_loss64(v, p, np.zeros(0), xf, tgt, grad=True)
+    assert st == 0
+    fun = lambda z: O.path_loss64(v, z.reshape(-1, 2), np.zeros(0), xf, tgt)[1]  # noqa: E731
+    for k in (2, 3):  # B.x, B.y
+        c, r, l = fd(fun, p.reshape(-1), k, 1e-5)
+        assert abs(r - l) <= 1e-3 * max(abs(c), 1.0), ("the loss should be smooth at the tie", k, r, l)
+        assert abs(dp.reshape(-1)[k] - c) <= 1e-4 * max(abs(c), 1.0), (k, dp.reshape(-1)[k], c)
