@@ -173,6 +173,27 @@ int ochre_rasterize_plain(ochre_context* ctx, const ochre_path_batch* in, ochre_
 int ochre_rasterize_batched(ochre_context* ctx, const ochre_path_batch* in, ochre_batch_output* out,
                             void* stream);
 
+/* ochre_rasterize_batched for callers that cannot allocate device memory
+ * (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc).  No reference counterpart.  It runs the dense kernels alone --
+ * the 64-tile instance over every path, then the 256-tile instance over the
+ * paths it hands on -- with their queues, counters and the list of handed-on
+ * paths in a fixed device array of 4 MiB + 256 B per device, held until the
+ * process ends; it calls no allocator (the caller passes the inputs and the
+ * outputs, as for ochre_rasterize_batched) and synchronises the stream once to
+ * read the counters back.  Limits: at most 2^20 paths per call
+ * (OCHRE_ERR_TOO_LARGE before anything runs); a path over the 256-tile
+ * instance (a tile grid of more than 256 tiles or 32 on a side, more than
+ * 128 lines over 8 pixel steps, a conic weight that is not positive and
+ * finite) makes the call return OCHRE_ERR_TOO_LARGE after the other paths are
+ * written, with that path's range unwritten (ochre_rasterize_batched handles
+ * it).  Every call on a device shares that device's array, which keeps the
+ * last call's contents; each call clears the counters it uses, and calls that
+ * overlap on one device race.  Results equal ochre_rasterize_batched with the
+ * dense kernels on: tile and span sets exact, alpha within 1 LSB. */
+int ochre_rasterize_batched_noalloc(ochre_context* ctx, const ochre_path_batch* in, ochre_batch_output* out,
+                                    void* stream);
+
 /* Rasterizer::stroke for every path of a batch (widths: [n_paths], device
  * memory): flattening and stroke expansion on the device (one wave per path),
  * then the plain or batched fill of the outlines with the paths' transforms.

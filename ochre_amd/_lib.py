@@ -33,7 +33,7 @@ EXPORTS = [
     "ochre_context_set_dense_large", "ochre_context_get_dense_large",
     "ochre_draw_plain", "ochre_draw_batched",
     "ochre_svg_parse", "ochre_svg_info", "ochre_svg_verbs", "ochre_svg_verb_offsets", "ochre_svg_points",
-    "ochre_svg_point_offsets", "ochre_svg_draws", "ochre_svg_free",
+    "ochre_svg_point_offsets", "ochre_svg_draws", "ochre_svg_free", "ochre_rasterize_batched_noalloc",
 ]
 
 
@@ -156,6 +156,9 @@ def _optional(lib):
                                                 ctypes.c_float, P]
         for name in ("ochre_stroke_plain", "ochre_stroke_batched"):
             getattr(lib, name).argtypes = [P, ctypes.POINTER(PathBatchC), P, ctypes.POINTER(BatchOutputC), P]
+    if hasattr(lib, "ochre_rasterize_batched_noalloc"):
+        lib.ochre_rasterize_batched_noalloc.argtypes = [P, ctypes.POINTER(PathBatchC), ctypes.POINTER(BatchOutputC),
+                                                        P]
     if hasattr(lib, "ochre_draw_plain"):
         for name in ("ochre_draw_plain", "ochre_draw_batched"):
             getattr(lib, name).argtypes = [P, ctypes.POINTER(PathBatchC), P, P, ctypes.POINTER(BatchOutputC), P]

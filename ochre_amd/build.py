@@ -16,7 +16,7 @@ INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(PKG, "libochre_hip.so")
 ARCH = os.environ.get("OCHRE_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["plain.hip", "batched.hip", "stroke.hip", "atlas.hip", "capi.hip", "svg.cpp"]
+SOURCES = ["plain.hip", "batched.hip", "stroke.hip", "atlas.hip", "noalloc.hip", "capi.hip", "svg.cpp"]
 
 # -ffp-contract=off: the reference's f32 arithmetic is never fused into FMAs.
 # -packed-fp32-ops: keep the compiler from emitting packed f32 VALU ops.

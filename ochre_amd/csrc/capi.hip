@@ -67,6 +67,9 @@ int ochre_context_create(int device, ochre_context** out) {
     ochre_context* c = new (std::nothrow) ochre_context();
     if (!c) return OCHRE_ERR_OUT_OF_MEMORY;
     c->c.device = device;
+    // the allocation-free entry point's scratch address on this device
+    // (ochre_rasterize_batched_noalloc fails with OCHRE_ERR_DEVICE without it)
+    if (hipSetDevice(device) == hipSuccess) (void)noalloc_init_device(device);
     *out = c;
     return OCHRE_OK;
 }
@@ -181,6 +184,21 @@ int ochre_rasterize_batched(ochre_context* ctx, const ochre_path_batch* in, ochr
     OutputDesc od = to_output(out);
     st = run_batched(&ctx->c, si, od, reinterpret_cast<hipStream_t>(stream), ctx->c.batched_waves_per_cu,
                      ctx->c.batched_gran, ctx->c.batched_tc, ctx->c.batched_graph);
+    out->n_tiles = od.n_tiles;
+    out->n_spans = od.n_spans;
+    return st;
+}
+
+int ochre_rasterize_batched_noalloc(ochre_context* ctx, const ochre_path_batch* in, ochre_batch_output* out,
+                                    void* stream) {
+    if (!ctx || !in || !out) return OCHRE_ERR_INVALID_ARG;
+    SegInput si;
+    int st = to_seg_input(in, &si);
+    if (st) return st;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (hipSetDevice(ctx->c.device) != hipSuccess) return OCHRE_ERR_DEVICE;
+    OutputDesc od = to_output(out);
+    st = run_batched_noalloc(si, od, reinterpret_cast<hipStream_t>(stream));
     out->n_tiles = od.n_tiles;
     out->n_spans = od.n_spans;
     return st;

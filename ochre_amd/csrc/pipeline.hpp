@@ -176,6 +176,11 @@ int run_batched(Context* ctx, const SegInput& in, OutputDesc& out, hipStream_t s
 constexpr size_t DENSE_CTL_BYTES = 256;
 int launch_dense_chain(const SegInput& in, const OutputDesc& out, uint8_t* ctl, uint32_t* ids, hipStream_t stream);
 int finish_dense_chain(const uint8_t* ctl, OutputDesc& out, hipStream_t stream);
+// The allocation-free batched call (noalloc.hip): the dense chain with its
+// control block and handed-on list in a fixed device array.  noalloc_init_device
+// looks the array's address up on the current device (once per device).
+int noalloc_init_device(int device);
+int run_batched_noalloc(const SegInput& in, OutputDesc& out, hipStream_t stream);
 
 // Target render of the coverage adjoint (device pointers, per path ranges).
 struct AdjTarget {
