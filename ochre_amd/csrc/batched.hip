@@ -129,6 +129,10 @@ struct LdsState {
             float acc_h[TC * F_TSTRIDE];
             float cin[TC * 8];
         } a;
+        struct {  // fixed-point area / height sums of half a chunk (the coverage adjoint)
+            unsigned long long fa[TC / 2 * 64];
+            unsigned long long fh[TC / 2 * 64];
+        } x;
     } u;
 };
 
@@ -215,7 +219,8 @@ constexpr size_t A_OFF_CMDOFF = A_OFF_CMD + (size_t)A_VCAP * 12 * 4;         // 
 constexpr size_t A_OFF_DW = A_OFF_CMDOFF + (size_t)(A_VCAP + 4) * 4;         // float2[A_PCAP]
 constexpr size_t A_OFF_PCMD = A_OFF_DW + (size_t)A_PCAP * 8;                 // uint32[A_PCAP]
 constexpr size_t A_OFF_SPANS = A_OFF_PCMD + (size_t)A_PCAP * 4;              // uint32[A_SCAP][2]
-constexpr size_t A_SCRATCH = A_OFF_SPANS + (size_t)A_SCAP * 8;
+constexpr size_t A_OFF_L64 = (A_OFF_SPANS + (size_t)A_SCAP * 8 + 255) & ~(size_t)255;  // double2[F_LCAP][2]
+constexpr size_t A_SCRATCH = A_OFF_L64 + (size_t)F_LCAP * 32;
 
 template <int TC>
 __device__ void adjoint_backward(const FusedIn& in, const AdjArgs& adj, FusedCtl* ctl, uint8_t* sc, uint8_t* asc,
@@ -1145,6 +1150,68 @@ int run_adjoint(Context* ctx, const SegInput& in, const AdjTarget& tgt, float* l
     if (hc->error & ERR_BBOX) return -3;
     if (hc->error & ERR_CAPACITY) return -7;
     return 0;
+}
+
+// ------------------------------------- a few paths, no host round trip
+//
+// The ordered kernel over a small batch (a Rasterizer's single fill): the
+// queue reset and one wave per path, launched directly, then an asynchronous
+// copy of the control block to `host_ctl` (pinned, SMALL_CTL_BYTES).  Nothing
+// synchronises; after the caller's synchronisation small_batch_result reads
+// the counts.  A path over the kernel's limits is only counted as deferred
+// (the caller reruns the batch through run_batched, which handles it).
+static_assert(sizeof(FusedCtl) == SMALL_CTL_BYTES, "control block size");
+
+int launch_small_batch(Context* ctx, const SegInput& in, const OutputDesc& out, hipStream_t s, void* host_ctl) {
+    if (!ctx->batched) ctx->batched = new BatchedState();
+    BatchedState* B = reinterpret_cast<BatchedState*>(ctx->batched);
+    const int64_t NP = in.n_paths;
+    if (NP <= 0) return 0;
+    const int n_waves = (int)std::min<int64_t>(NP, 1024);
+    HIPCHK_RET(B->scratch.ensure((size_t)n_waves * F_SCRATCH));
+    HIPCHK_RET(B->ctl.ensure(sizeof(FusedCtl)));
+    HIPCHK_RET(B->deferred.ensure((size_t)NP * 4));
+    FusedIn fi;
+    fi.n_paths = NP;
+    fi.verbs = in.verbs;
+    fi.verb_off = in.seg_verb_off;
+    fi.points = in.points;
+    fi.point_off = in.seg_point_off;
+    fi.weights = in.weights;
+    fi.weight_off = in.seg_weight_off;
+    fi.xf = in.seg_xf;
+    fi.n_points = in.n_points;
+    fi.n_weights = in.n_weights;
+    FusedOut fo;
+    fo.tile_xy = out.tile_xy;
+    fo.tile_alpha = out.tile_alpha;
+    fo.tile_cov = out.tile_cov;
+    fo.tile_cap = out.tile_capacity;
+    fo.span_xy = out.span_xy;
+    fo.span_w = out.span_w;
+    fo.span_cap = out.span_capacity;
+    fo.pts = out.path_tile_start;
+    fo.ptc = out.path_tile_count;
+    fo.pss = out.path_span_start;
+    fo.psc = out.path_span_count;
+    FusedCtl* ctl = B->ctl.as<FusedCtl>();
+    hipLaunchKernelGGL(k_fused_reset, dim3(1), dim3(64), 0, s, ctl);
+    hipLaunchKernelGGL(k_fused<32>, dim3(n_waves), dim3(WAVE), 0, s, fi, fo, ctl, B->deferred.as<uint32_t>(),
+                       B->scratch.as<uint8_t>(), 1u);
+    HIPCHK_RET(hipGetLastError());
+    HIPCHK_RET(hipMemcpyAsync(host_ctl, ctl, sizeof(FusedCtl), hipMemcpyDeviceToHost, s));
+    return 0;
+}
+
+SmallBatchResult small_batch_result(const void* host_ctl) {
+    const FusedCtl* hc = reinterpret_cast<const FusedCtl*>(host_ctl);
+    SmallBatchResult r;
+    r.tiles = (int64_t)hc->tiles;
+    r.spans = (int64_t)hc->spans;
+    r.deferred = hc->deferred;
+    r.status = (hc->error & ERR_INPUT) ? -1 : (hc->error & ERR_FLATTEN) ? -5 : (hc->error & ERR_BBOX) ? -3
+               : (hc->error & ERR_CAPACITY) ? -4 : 0;
+    return r;
 }
 
 // ------------------------------------------------ the dense chain alone

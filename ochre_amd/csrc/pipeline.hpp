@@ -167,6 +167,20 @@ int run_plain(Context* ctx, const SegInput& in, OutputDesc& out, hipStream_t str
 // a captured HIP graph.
 int run_batched(Context* ctx, const SegInput& in, OutputDesc& out, hipStream_t stream, int waves_per_cu, int gran,
                 int tiles_per_chunk, bool use_graph);
+// The ordered kernel over a small batch with no host round trip: launches,
+// then copies the control block (SMALL_CTL_BYTES) to the pinned `host_ctl`
+// asynchronously; after synchronising, small_batch_result gives the counts,
+// the status of run_batched and how many paths were over the kernel's limits
+// (their output is not written: rerun through run_batched).
+constexpr size_t SMALL_CTL_BYTES = 256;
+struct SmallBatchResult {
+    int64_t tiles, spans;
+    uint32_t deferred;
+    int status;
+};
+int launch_small_batch(Context* ctx, const SegInput& in, const OutputDesc& out, hipStream_t stream, void* host_ctl);
+SmallBatchResult small_batch_result(const void* host_ctl);
+
 // The dense kernels alone (the 64-tile instance, then the 256-tile one over
 // what it hands on) with caller-provided device memory: `ctl` of
 // DENSE_CTL_BYTES (cleared by a kernel at the start) and `ids` of n_paths

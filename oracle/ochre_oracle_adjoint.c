@@ -356,7 +356,8 @@ typedef struct {
     int k0, k1;
     double b0, b1;
     double area, height;
-    int virt; /* a zero-length derivative carrier (see walk): no tile, no sums */
+    int virt;   /* a zero-length derivative carrier (see walk): no tile, no sums */
+    int nograd; /* part of the forward, but no derivative (see walk) */
 } incr;
 
 typedef struct {
@@ -396,6 +397,7 @@ static void walk(d2 last, d2 point, incrs* I, tincs* T, int16_t* ty_prev) {
     double xs = fabs(dtdx), ys = fabs(dtdy);
     int k_prev = 0;    /* kind of the bound of t0 */
     double b_prev = 0; /* its boundary coordinate */
+    int nograd = 0;    /* set once the walk has passed the line's end (see below) */
     for (;;) {
         double t0 = fmax(rt0, ct0), t1 = fmin(rt1, ct1);
         int k1;
@@ -418,7 +420,7 @@ static void walk(d2 last, d2 point, incrs* I, tincs* T, int16_t* ty_prev) {
          * derivatives on the same side of that tie: this increment ends at
          * the boundary, and a zero-length increment in the pixel beyond
          * carries the end point's derivative.  (A corner tie keeps the end.) */
-        int vx = 0, vy = 0;
+        int vx = 0, vy = 0, ends_here = 0;
         if (k1 == 0) {
             int on_y = yd > 0 && point.y == (double)(int16_t)(y + 1);
             int on_x = xd > 0 && point.x == (double)(int16_t)(x + 1);
@@ -426,6 +428,14 @@ static void walk(d2 last, d2 point, incrs* I, tincs* T, int16_t* ty_prev) {
                 if (on_y) { k1 = 1; b1 = (double)(int16_t)(y + 1); vy = 1; }
                 else { k1 = 2; b1 = (double)(int16_t)(x + 1); vx = 1; }
             }
+        } else if (k1 == 1 && yd < 0 && point.y == b1) {
+            /* Going up onto a row boundary that holds the line's end point, the
+             * crossing parameter can round below 1: the walk then steps into the
+             * row above with a sliver, while the next line starts in this row.
+             * On the same (floor) side, this increment ends at the end point and
+             * the sliver carries no derivative. */
+            k1 = 0;
+            ends_here = 1;
         }
         incr* in = &I->v[I->n++];
         in->x = x;
@@ -439,7 +449,9 @@ static void walk(d2 last, d2 point, incrs* I, tincs* T, int16_t* ty_prev) {
         in->area = area;
         in->height = h;
         in->virt = 0;
-        if (vx || vy) {
+        in->nograd = nograd;
+        if (ends_here) nograd = 1;
+        if ((vx || vy) && !nograd) {
             if (I->n == I->cap) {
                 I->cap = 2 * I->cap;
                 I->v = (incr*)realloc(I->v, I->cap * sizeof(incr));
@@ -457,6 +469,7 @@ static void walk(d2 last, d2 point, incrs* I, tincs* T, int16_t* ty_prev) {
             vi->area = 0.0;
             vi->height = 0.0;
             vi->virt = 1;
+            vi->nograd = 0;
         }
         if (rt1 < ct1) {
             rt0 = rt1;
@@ -834,6 +847,7 @@ int orc64_path_loss(const uint8_t* verbs, size_t nv, const double* points, size_
         d2* dW = (d2*)calloc(np + 1, sizeof(d2));
         for (size_t i = 0; i < I.n; i++) {
             const incr* in = &I.v[i];
+            if (in->nograd) continue;
             const line* ln = &L.v[line_of[i]];
             long k = find_tile(&R, div8(in->x), div8(in->y));
             if (k < 0) continue; /* a zero-length carrier outside the path's tiles */

@@ -59,6 +59,7 @@ def _check_adjoint(ctx, b, tb, fd_samples, seed):
     targets = [host_target(tgt, i) for i in range(b.n_paths)]
     errs = []
     worst = (0.0, None)
+    beyond = []  # elements off the f64 gradient by more than 1e-3
     for i in range(b.n_paths):
         v, p, w, xf = b.path(i)
         st, L, gp, gx, _ = O.path_loss64(v, p.astype(np.float64), w, xf.astype(np.float64), targets[i], grad=True)
@@ -77,11 +78,42 @@ def _check_adjoint(ctx, b, tb, fd_samples, seed):
         k = int(np.argmax(rel))
         if rel[k] > worst[0]:
             worst = (float(rel[k]), (i, k, float(got[k]), float(ref[k])))
+        beyond += [(i, int(k), float(got[k]), float(ref[k])) for k in np.nonzero(rel > 1e-3)[0]]
     errs = np.concatenate(errs)
     print(f"adjoint {b.name}: {errs.size} gradient elements, per-parameter rel err vs f64 adjoint: "
           f"median {np.median(errs):.1e}, p99.9 {np.quantile(errs, 0.999):.1e}, max {worst[0]:.1e} at {worst[1]}",
           flush=True)
-    assert worst[0] <= 1e-3, worst
+    # The device takes the forward's discrete decisions in f32 (which pixels a
+    # line visits, in the local frame) on end points that lie within ~1e-4 px
+    # of the f64 ones.  Where a kink of the loss lies that close (a y-extremal
+    # vertex on a pixel-row boundary, a sliver taking |v| to 0 or 1), the
+    # device's gradient is the one-sided derivative of the other side.  Such an
+    # element must lie between the f64 forward's one-sided difference
+    # quotients at h = 1e-4 px, which must disagree (a kink), and there may be
+    # only a handful of them.
+    for i, k, g_dev, g_ref in beyond:
+        v, p, w, xf = b.path(i)
+        p64, xf64, t = p.astype(np.float64), xf.astype(np.float64), targets[i]
+        if k < p64.size:
+            h = 1e-4 / float(xf64[0])
+            x0 = p64.reshape(-1).copy()
+            f = lambda z: O.path_loss64(v, z.reshape(-1, 2), w, xf64, t)[1]  # noqa: E731
+        else:
+            h = 1e-4 / 32.0 if k - p64.size < 4 else 1e-4
+            x0 = xf64.copy()
+            f = lambda z: O.path_loss64(v, p64, w, z, t)[1]  # noqa: E731
+        kk = k if k < p64.size else k - p64.size
+        xp, xm = x0.copy(), x0.copy()
+        xp[kk] += h
+        xm[kk] -= h
+        lp, lm, l0 = f(xp), f(xm), f(x0)
+        r, l = (lp - l0) / h, (l0 - lm) / h
+        tol = 1e-3 * max(abs(g_ref), 1.0)
+        print(f"  path {i} element {k}: device {g_dev:.6g}, f64 {g_ref:.6g}, one-sided f64 at 1e-4 px "
+              f"{l:.6g} / {r:.6g}", flush=True)
+        assert abs(r - l) > 2 * tol, (i, k, g_dev, g_ref, l, r)
+        assert min(l, r) - tol <= g_dev <= max(l, r) + tol, (i, k, g_dev, g_ref, l, r)
+    assert len(beyond) <= max(4, errs.size // 100000), len(beyond)
     # central finite differences of the f64 forward (h = 1e-3 px) on seeded
     # (path, parameter) samples; a sample whose [x - h, x + h] holds a kink
     # (the one-sided quotients disagree by more than the curvature of a smooth

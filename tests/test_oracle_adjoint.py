@@ -78,7 +78,7 @@ def test_adjoint_matches_finite_differences(seed):
     assert checked >= 20
 
 
-@pytest.mark.parametrize("axis", [0, 1])
+@pytest.mark.parametrize("axis", [0, 1, "up"])
 def test_adjoint_at_a_vertex_on_a_pixel_boundary(axis):
     """A vertex lying exactly on a pixel boundary inside a monotone edge: the
     coverage is smooth there (the edge crosses the boundary once whichever
@@ -88,6 +88,10 @@ def test_adjoint_at_a_vertex_on_a_pixel_boundary(axis):
     derivatives have to be taken on the same side of the tie."""
     if axis == 1:  # B on the row boundary y = 12, both edges going down
         pts = [(3.3, 2.6), (7.45, 12.0), (9.8, 21.3), (20.2, 19.7), (18.1, 4.4)]
+    elif axis == "up":  # both edges going up; the walk of A -> B crosses y = 12
+        # at t = 1 - 2^-53 (the f64 product (1 / (12 - 12.35)) * (12 - 12.35))
+        # and ends in the row above with a sliver of zero height
+        pts = [(3.3, 12.35), (7.45, 12.0), (9.8, 2.6), (20.2, 4.3), (18.1, 19.6)]
     else:  # B on the column boundary x = 12, both edges going right
         pts = [(2.6, 3.3), (12.0, 7.45), (21.3, 9.8), (19.7, 20.2), (4.4, 18.1)]
     v = np.array([0, 1, 1, 1, 1, 5], np.uint8)
