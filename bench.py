@@ -195,6 +195,59 @@ def api_latency(reps=20):
                     "ochre_rasterizer_finish with a NULL builder (C ABI, no callbacks); medians"}
 
 
+def svg_demo_latency(reps=3):
+    """The demo's loop on the Ghostscript tiger (examples/svg.rs:140-156): a
+    Rasterizer per fill or stroke, each finished into the builder, through the
+    C ABI with a NULL builder (the library's own cost: upload, device work,
+    read-back): one ochre_rasterizer_finish per draw, then all of them in one
+    ochre_rasterizer_finish_many.  Medians of `reps` passes."""
+    import gzip
+    from ochre_amd import _lib, svg
+    path = os.path.join(ROOT, "tests", "golden", "Ghostscript_Tiger.svg.gz")
+    if not os.path.exists(path):
+        return None
+    b = svg.draw_batch(svg.parse(gzip.open(path).read()))
+    lib = _lib.load()
+    P = ctypes.c_void_p
+
+    def make():
+        hs = []
+        for i in range(b.n_paths):
+            v, p, w, xf = b.path(i)
+            h = lib.ochre_rasterizer_new()
+            t = _lib.Transform((ctypes.c_float * 4)(*xf[:4]), (ctypes.c_float * 2)(*xf[4:]))
+            vv, pp = v.ctypes.data_as(P), p.ctypes.data_as(P)
+            if b.strokes[i]:
+                st = lib.ochre_rasterizer_stroke(h, vv, v.size, pp, p.shape[0], None, 0, float(b.widths[i]),
+                                                 ctypes.byref(t))
+            else:
+                st = lib.ochre_rasterizer_fill(h, vv, v.size, pp, p.shape[0], None, 0, ctypes.byref(t))
+            _lib.check(st, "fill/stroke")
+            hs.append(h)
+        return hs
+
+    one, many = [], []
+    for _ in range(reps + 1):
+        hs = make()
+        t = time.perf_counter()
+        for h in hs:
+            _lib.check(lib.ochre_rasterizer_finish(h, None), "finish")
+        one.append(time.perf_counter() - t)
+        hs = make()
+        arr = (P * len(hs))(*hs)
+        t = time.perf_counter()
+        _lib.check(lib.ochre_rasterizer_finish_many(arr, len(hs), None), "finish_many")
+        many.append(time.perf_counter() - t)
+    one, many = sorted(one[1:]), sorted(many[1:])
+    n = b.n_paths
+    return {"draws": n, "strokes": int(b.strokes.sum()),
+            "finish_each_ms": one[len(one) // 2] * 1e3, "finish_each_us_per_draw": one[len(one) // 2] / n * 1e6,
+            "finish_many_ms": many[len(many) // 2] * 1e3,
+            "finish_many_us_per_draw": many[len(many) // 2] / n * 1e6,
+            "what": "the tiger's draw list (examples/res/Ghostscript_Tiger.svg as a fixture), one Rasterizer per "
+                    "draw, NULL TileBuilder; medians"}
+
+
 def plain_per_path(ctx, batch, n=200):
     """The plain pipeline launched once per path (the drop-in use: one
     Rasterizer per path, as examples/svg.rs:143-145 runs it), device-resident
@@ -674,6 +727,10 @@ def main():
         "roofline": roofline(kname, res[head]["kernel_ms"], ms, alg_k, b_in + b_out, batch.name),
     }
     line["rasterizer_api_latency"] = api_latency()
+    try:
+        line["svg_demo_latency"] = svg_demo_latency()
+    except Exception as e:  # report the bench line without it
+        log(f"svg demo latency probe skipped: {e}")
     if not stroking:
         try:
             pp = plain_per_path(ctx, batch)

@@ -103,6 +103,15 @@ int ochre_rasterizer_stroke(ochre_rasterizer* r, const uint8_t* verbs, size_t n_
 /* Rasterizer::finish: rasterises on the GPU, then calls the builder.
  * Consumes the rasterizer (the handle is freed, whatever the status). */
 int ochre_rasterizer_finish(ochre_rasterizer* r, const ochre_tile_builder* builder);
+/* Finishes n rasterizers into one builder, in order: the calls builder
+ * receives are those of ochre_rasterizer_finish on each in turn (the demo's
+ * loop, one Rasterizer per fill or stroke finishing into one Builder,
+ * examples/svg.rs:140-156).  Consecutive rasterizers that each hold one fill,
+ * one stroke or nothing are rasterised together in one device call (the
+ * ordered, bit-exact kernels); others are finished one at a time.  Consumes
+ * every handle; returns the first error (the remaining rasterizers are still
+ * finished).  No single reference counterpart. */
+int ochre_rasterizer_finish_many(ochre_rasterizer* const* rasterizers, size_t n, const ochre_tile_builder* builder);
 /* Frees a rasterizer that will not be finished. */
 void ochre_rasterizer_free(ochre_rasterizer* r);
 
@@ -316,6 +325,19 @@ typedef struct ochre_batched_tuning {
 } ochre_batched_tuning;
 int ochre_autotune_batched(ochre_context* ctx, const ochre_path_batch* in, ochre_batch_output* out, void* stream,
                            int reps, ochre_batched_tuning* results, int max_results, int* n_results);
+
+/* The autotuner with a cache.  The key is the device (its architecture name
+ * and compute unit count), whether the dense kernels are on, and a coarse
+ * signature of the batch (power-of-two buckets of its path count and
+ * half-octave buckets of its mean commands and points per path).  A key
+ * found in the process's table -- which first reads `cache_path`, when not
+ * NULL, once per path -- sets the stored parameters in the context without
+ * timing anything (*tuned = 0); otherwise the batch is tuned as by
+ * ochre_autotune_batched (reps calls per configuration), the choice stored,
+ * and, with a cache path, appended to that file as one text line (*tuned = 1).
+ * The cache is the library's: every caller of the C ABI shares it. */
+int ochre_autotune_batched_cached(ochre_context* ctx, const ochre_path_batch* in, ochre_batch_output* out,
+                                  void* stream, int reps, const char* cache_path, int* tuned);
 
 /* Kernel-time accounting for measurement: when enabled, the pipeline
  * records HIP events around its dominant kernel; ochre_context_timing

@@ -243,6 +243,29 @@ class Rasterizer:
         _lib.check(self._lib.ochre_rasterizer_finish(h, ctypes.byref(tb)), "finish")
 
 
+def finish_many(rasterizers, builder: TileBuilder):
+    """Finishes the rasterizers into one builder, in order (the demo's loop
+    of one Rasterizer per fill or stroke, examples/svg.rs:140-156); those that
+    hold one fill or one stroke are rasterised together in one device call.
+    Consumes every rasterizer."""
+    lib = _lib.load()
+
+    def on_tile(_u, x, y, a):
+        builder.tile(int(x), int(y), np.ctypeslib.as_array(a, shape=(64,)).copy())
+
+    def on_span(_u, x, y, w):
+        builder.span(int(x), int(y), int(w))
+
+    tf, sf = _lib.TILE_FN(on_tile), _lib.SPAN_FN(on_span)
+    tb = _lib.TileBuilderC(None, ctypes.cast(tf, ctypes.c_void_p), ctypes.cast(sf, ctypes.c_void_p))
+    hs = []
+    for r in rasterizers:
+        hs.append(r._h)
+        r._h = None
+    arr = (ctypes.c_void_p * max(len(hs), 1))(*hs)
+    _lib.check(lib.ochre_rasterizer_finish_many(arr, len(hs), ctypes.byref(tb)), "finish_many")
+
+
 class EventRecorder(TileBuilder):
     """A TileBuilder that records the call sequence."""
 
@@ -257,4 +280,4 @@ class EventRecorder(TileBuilder):
 
 
 __all__ = ["TILE_SIZE", "Vec2", "Mat2x2", "Transform", "PathCmd", "TileBuilder", "Rasterizer",
-           "EventRecorder", "OchreError", "encode_path"]
+           "EventRecorder", "OchreError", "encode_path", "finish_many"]

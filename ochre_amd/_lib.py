@@ -34,6 +34,7 @@ EXPORTS = [
     "ochre_draw_plain", "ochre_draw_batched",
     "ochre_svg_parse", "ochre_svg_info", "ochre_svg_verbs", "ochre_svg_verb_offsets", "ochre_svg_points",
     "ochre_svg_point_offsets", "ochre_svg_draws", "ochre_svg_free", "ochre_rasterize_batched_noalloc",
+    "ochre_autotune_batched_cached", "ochre_rasterizer_finish_many",
 ]
 
 
@@ -109,6 +110,8 @@ def load() -> ctypes.CDLL:
     lib.ochre_rasterizer_command.argtypes = [P, ctypes.c_uint8, P, f32]
     lib.ochre_rasterizer_fill.argtypes = [P, P, ctypes.c_size_t, P, ctypes.c_size_t, P, ctypes.c_size_t, P]
     lib.ochre_rasterizer_finish.argtypes = [P, P]
+    if hasattr(lib, "ochre_rasterizer_finish_many"):
+        lib.ochre_rasterizer_finish_many.argtypes = [ctypes.POINTER(P), ctypes.c_size_t, P]
     lib.ochre_rasterizer_free.argtypes = [P]
     lib.ochre_rasterizer_free.restype = None
     lib.ochre_context_create.argtypes = [ctypes.c_int, ctypes.POINTER(P)]
@@ -136,6 +139,9 @@ def _optional(lib):
                                                P, P]
         lib.ochre_autotune_batched.argtypes = [P, ctypes.POINTER(PathBatchC), ctypes.POINTER(BatchOutputC), P, I,
                                                ctypes.POINTER(BatchedTuningC), I, ctypes.POINTER(I)]
+    if hasattr(lib, "ochre_autotune_batched_cached"):
+        lib.ochre_autotune_batched_cached.argtypes = [P, ctypes.POINTER(PathBatchC), ctypes.POINTER(BatchOutputC),
+                                                      P, ctypes.c_int, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int)]
     if hasattr(lib, "ochre_context_set_dense_params"):
         I = ctypes.c_int
         lib.ochre_context_set_dense_params.argtypes = [P, I, I, I]

@@ -10,10 +10,14 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <new>
+#include <set>
+#include <string>
 #include <vector>
 
 #include "../../include/ochre.h"
@@ -484,6 +488,116 @@ int ochre_autotune_batched(ochre_context* ctx, const ochre_path_batch* in, ochre
 
 }  // extern "C"
 
+// ------------------------------------------------------- autotune cache
+//
+// Tuned parameters keyed by the device (its name and compute unit count), the
+// dense kernels' setting and a coarse signature of the batch: the
+// power-of-two buckets of its path count and of its mean commands and points
+// per path (half-octave steps).  One table per process; a cache file holds
+// one "key = values" line per entry and is read once per path.
+namespace {
+
+struct TuneEntry {
+    int waves_per_cu, gran, tc, dense_groups, dense_waves, dense_small, dense_large;
+};
+
+std::mutex g_tune_mu;
+std::map<std::string, TuneEntry> g_tune;
+std::set<std::string> g_tune_loaded;
+
+int bucket(double v, double per_octave) { return v > 0 ? (int)std::floor(std::log2(v) * per_octave) : -1; }
+
+std::string tune_key(const Context& c, const ochre_path_batch* in) {
+    hipDeviceProp_t prop;
+    std::string name = "unknown";
+    int cus = 0;
+    if (hipGetDeviceProperties(&prop, c.device) == hipSuccess) {
+        name = prop.gcnArchName;
+        cus = prop.multiProcessorCount;
+    }
+    for (char& ch : name)
+        if (ch == ' ' || ch == '=') ch = '_';
+    const double np = (double)std::max<int64_t>(in->n_paths, 1);
+    char buf[256];
+    snprintf(buf, sizeof buf, "%s/cu%d/dense%d/paths%d/verbs%d/points%d", name.c_str(), cus, c.dense ? 1 : 0,
+             bucket((double)in->n_paths, 1.0), bucket((double)in->n_verbs / np, 2.0),
+             bucket((double)in->n_points / np, 2.0));
+    return buf;
+}
+
+void load_tune_file(const char* path) {
+    if (!path || g_tune_loaded.count(path)) return;
+    g_tune_loaded.insert(path);
+    FILE* f = fopen(path, "r");
+    if (!f) return;
+    char line[512];
+    while (fgets(line, sizeof line, f)) {
+        char key[320];
+        TuneEntry e;
+        if (sscanf(line, "%319s = %d %d %d %d %d %d %d", key, &e.waves_per_cu, &e.gran, &e.tc, &e.dense_groups,
+                   &e.dense_waves, &e.dense_small, &e.dense_large) == 8)
+            g_tune[key] = e;
+    }
+    fclose(f);
+}
+
+}  // namespace
+
+extern "C" {
+
+int ochre_autotune_batched_cached(ochre_context* ctx, const ochre_path_batch* in, ochre_batch_output* out,
+                                  void* stream, int reps, const char* cache_path, int* tuned) {
+    if (!ctx || !in || !out || reps < 1) return OCHRE_ERR_INVALID_ARG;
+    if (tuned) *tuned = 0;
+    std::string key;
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        key = tune_key(ctx->c, in);
+    }
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        load_tune_file(cache_path);
+        auto it = g_tune.find(key);
+        if (it != g_tune.end()) {
+            const TuneEntry& e = it->second;
+            std::lock_guard<std::mutex> lk2(ctx->mu);
+            ctx->c.batched_waves_per_cu = e.waves_per_cu;
+            ctx->c.batched_gran = e.gran;
+            ctx->c.batched_tc = e.tc;
+            ctx->c.batched_graph = true;
+            if (ctx->c.dense) {
+                ctx->c.dense_groups_per_cu = e.dense_groups;
+                ctx->c.dense_waves = e.dense_waves;
+                ctx->c.dense_small_groups = e.dense_small;
+                ctx->c.dense_large = e.dense_large;
+            }
+            return OCHRE_OK;
+        }
+    }
+    int st = ochre_autotune_batched(ctx, in, out, stream, reps, nullptr, 0, nullptr);
+    if (st) return st;
+    TuneEntry e;
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        e = TuneEntry{ctx->c.batched_waves_per_cu, ctx->c.batched_gran, ctx->c.batched_tc,
+                      ctx->c.dense_groups_per_cu, ctx->c.dense_waves, ctx->c.dense_small_groups, ctx->c.dense_large};
+    }
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune[key] = e;
+    if (cache_path) {
+        FILE* f = fopen(cache_path, "a");
+        if (f) {
+            fprintf(f, "%s = %d %d %d %d %d %d %d\n", key.c_str(), e.waves_per_cu, e.gran, e.tc, e.dense_groups,
+                    e.dense_waves, e.dense_small, e.dense_large);
+            fclose(f);
+        }
+    }
+    if (tuned) *tuned = 1;
+    return OCHRE_OK;
+}
+
+}  // extern "C"
+
 // --------------------------------------------------------------- Rasterizer
 
 struct ochre_rasterizer {
@@ -545,8 +659,15 @@ struct DevPath {
     void* stage_host = nullptr;
     size_t stage_cap = 0;
     DBuf stage_dev;
+    // a single fill's output in one device arena, read back with one copy
+    // into a pinned one (whose tail also receives the kernel's control block)
+    DBuf arena;
+    void* arena_host = nullptr;
+    size_t arena_host_cap = 0;
+    int64_t fast_tiles = 256, fast_spans = 64;  // the arena's capacities
     ~DevPath() {
         if (stage_host) (void)hipHostFree(stage_host);
+        if (arena_host) (void)hipHostFree(arena_host);
     }
 };
 
@@ -760,6 +881,71 @@ int ochre_rasterizer_fill(ochre_rasterizer* r, const uint8_t* verbs, size_t n_ve
     return OCHRE_OK;
 }
 
+// The TileBuilder replay in the reference's call order: a span follows the
+// tile to its left (rasterizer.rs:241,263).
+static void replay(const ochre_tile_builder* builder, const int16_t* txy, const uint8_t* ta, int64_t nt,
+                   const int16_t* sxy, const uint16_t* sw, int64_t ns) {
+    if (!builder) return;
+    int64_t k = 0;
+    for (int64_t t = 0; t < nt; t++) {
+        const int16_t x = txy[2 * t], y = txy[2 * t + 1];
+        if (builder->tile) builder->tile(builder->user, x, y, ta + (size_t)t * 64);
+        while (k < ns && sxy[2 * k + 1] == y && sxy[2 * k] == (int16_t)(x + 8)) {
+            if (builder->span) builder->span(builder->user, sxy[2 * k], sxy[2 * k + 1], sw[k]);
+            k++;
+        }
+    }
+}
+
+// A Rasterizer holding one fill: one upload (already queued), two launches,
+// one read-back of the control block and of the whole output arena, one
+// synchronisation, then the replay.  Returns 1 when the fill needs the
+// general path (a path over the ordered kernel's limits, or more output than
+// the arena holds: its capacities grow for the next call).
+static int finish_one_fill(ochre_context* ctx, DevPath* dp, const SegInput& si, hipStream_t s,
+                           const ochre_tile_builder* builder) {
+    const int64_t tc = dp->fast_tiles, sc = dp->fast_spans;
+    const size_t o_txy = (size_t)tc * 64, o_sxy = o_txy + (size_t)tc * 4, o_sw = o_sxy + (size_t)sc * 4;
+    const size_t o_rng = (o_sw + (size_t)sc * 2 + 255) & ~(size_t)255, total = o_rng + 256;
+    hipError_t e = dp->arena.ensure(total);
+    if (e != hipSuccess) return map_hip_error(e);
+    if (dp->arena_host_cap < total + SMALL_CTL_BYTES) {
+        if (dp->arena_host) (void)hipHostFree(dp->arena_host);
+        dp->arena_host = nullptr;
+        dp->arena_host_cap = 0;
+        if (hipHostMalloc(&dp->arena_host, total + SMALL_CTL_BYTES) != hipSuccess) return OCHRE_ERR_OUT_OF_MEMORY;
+        dp->arena_host_cap = total + SMALL_CTL_BYTES;
+    }
+    uint8_t* a = dp->arena.as<uint8_t>();
+    OutputDesc od;
+    od.tile_alpha = a;
+    od.tile_xy = reinterpret_cast<int16_t*>(a + o_txy);
+    od.tile_capacity = tc;
+    od.span_xy = reinterpret_cast<int16_t*>(a + o_sxy);
+    od.span_w = reinterpret_cast<uint16_t*>(a + o_sw);
+    od.span_capacity = sc;
+    od.path_tile_start = reinterpret_cast<int64_t*>(a + o_rng);
+    od.path_tile_count = reinterpret_cast<int32_t*>(a + o_rng + 8);
+    od.path_span_start = reinterpret_cast<int64_t*>(a + o_rng + 16);
+    od.path_span_count = reinterpret_cast<int32_t*>(a + o_rng + 24);
+    uint8_t* h = static_cast<uint8_t*>(dp->arena_host);
+    int st = launch_small_batch(&ctx->c, si, od, s, h + total);
+    if (st) return st;
+    if ((e = hipMemcpyAsync(h, a, total, hipMemcpyDeviceToHost, s)) != hipSuccess) return map_hip_error(e);
+    if ((e = hipStreamSynchronize(s)) != hipSuccess) return map_hip_error(e);
+    const SmallBatchResult res = small_batch_result(h + total);
+    if (res.status == OCHRE_ERR_CAPACITY || res.tiles > tc || res.spans > sc) {
+        dp->fast_tiles = std::max<int64_t>(tc, res.tiles + res.tiles / 4 + 64);
+        dp->fast_spans = std::max<int64_t>(sc, res.spans + res.spans / 4 + 64);
+        return 1;
+    }
+    if (res.status) return res.status;
+    if (res.deferred) return 1;
+    replay(builder, reinterpret_cast<const int16_t*>(h + o_txy), h, res.tiles,
+           reinterpret_cast<const int16_t*>(h + o_sxy), reinterpret_cast<const uint16_t*>(h + o_sw), res.spans);
+    return 0;
+}
+
 int ochre_rasterizer_finish(ochre_rasterizer* r, const ochre_tile_builder* builder) {
     if (!r) return OCHRE_ERR_INVALID_ARG;
     ochre_context* ctx = r->ctx;
@@ -821,6 +1007,14 @@ int ochre_rasterizer_finish(ochre_rasterizer* r, const ochre_tile_builder* build
         si.n_weights = (int64_t)r->weights.size();
         OutputDesc od;
         const bool one_fill = nseg == 1 && !r->seg_raw[0] && r->s_width.empty();
+        if (one_fill && !ctx->c.timing) {
+            SegInput s1 = si;
+            s1.seg_path = nullptr;
+            s1.path_seg_off = nullptr;
+            st = finish_one_fill(ctx, dp, s1, s, builder);
+            if (st != 1) break;  // done (or an error); 1: the general path below
+            st = 0;
+        }
         int64_t tcap = (int64_t)(dp->tile_alpha.bytes / 64), scap = (int64_t)(dp->span_w.bytes / 2);
         for (int attempt = 0; attempt < 2; attempt++) {
             hipError_t e;
@@ -883,22 +1077,161 @@ int ochre_rasterizer_finish(ochre_rasterizer* r, const ochre_tile_builder* build
             st = map_hip_error(e);
             break;
         }
-        // TileBuilder replay in the reference's call order: a span follows
-        // the tile to its left (rasterizer.rs:241,263).
-        if (builder) {
-            size_t si_ = 0;
-            for (int64_t t = 0; t < od.n_tiles; t++) {
-                int16_t x = txy[2 * t], y = txy[2 * t + 1];
-                if (builder->tile) builder->tile(builder->user, x, y, &ta[(size_t)t * 64]);
-                while (si_ < (size_t)od.n_spans && sxy[2 * si_ + 1] == y && sxy[2 * si_] == (int16_t)(x + 8)) {
-                    if (builder->span) builder->span(builder->user, sxy[2 * si_], sxy[2 * si_ + 1], sw[si_]);
-                    si_++;
-                }
-            }
-        }
+        replay(builder, txy.data(), ta.data(), od.n_tiles, sxy.data(), sw.data(), od.n_spans);
     } while (0);
     delete r;
     return st;
+}
+
+// ------------------------------------------------------- finish_many
+//
+// Rasterizers holding exactly one fill or one stroke (or nothing) -- the
+// demo's pattern, one Rasterizer per fill or stroke finishing into one
+// Builder (examples/svg.rs:140-156) -- are rasterised together: one upload,
+// one draw-list call through the ordered (bit-exact) kernels, one read-back,
+// then each one's TileBuilder calls in order.
+static bool single_draw(const ochre_rasterizer* r) {
+    if (r->seg_xf.empty()) return true;  // empty: one zero tile
+    if (r->seg_xf.size() != 1 || r->seg_raw[0]) return false;
+    return r->s_width.empty() || (r->s_width.size() == 1 && r->seg_stroke[0] == 0);
+}
+
+static int finish_draws(ochre_rasterizer* const* rs, size_t m, const ochre_tile_builder* builder) {
+    ochre_context* ctx = rs[0]->ctx;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    static DevPath* dp = nullptr;  // guarded by the default context's mutex
+    if (!dp) dp = new (std::nothrow) DevPath();
+    if (!dp) return OCHRE_ERR_OUT_OF_MEMORY;
+    if (hipSetDevice(ctx->c.device) != hipSuccess) return OCHRE_ERR_DEVICE;
+    hipStream_t s = nullptr;
+    std::vector<uint8_t> verbs, flags(m, 0);
+    std::vector<float> points, weights, widths(m, 0.0f);
+    std::vector<int64_t> voff{0}, poff{0}, woff{0};
+    std::vector<Xform> xf(m);
+    bool any_stroke = false;
+    for (size_t k = 0; k < m; k++) {
+        const ochre_rasterizer* r = rs[k];
+        xf[k] = r->seg_xf.empty() ? Xform{1, 0, 0, 1, 0, 0} : r->seg_xf[0];
+        const bool stroke = !r->s_width.empty();
+        const std::vector<uint8_t>& v = stroke ? r->s_verbs : r->verbs;
+        const std::vector<float>& pt = stroke ? r->s_points : r->points;
+        const std::vector<float>& w = stroke ? r->s_weights : r->weights;
+        verbs.insert(verbs.end(), v.begin(), v.end());
+        points.insert(points.end(), pt.begin(), pt.end());
+        weights.insert(weights.end(), w.begin(), w.end());
+        voff.push_back((int64_t)verbs.size());
+        poff.push_back((int64_t)points.size() / 2);
+        woff.push_back((int64_t)weights.size());
+        if (stroke) {
+            flags[k] = 1;
+            widths[k] = r->s_width[0];
+            any_stroke = true;
+        }
+    }
+    const HostPiece pc[9] = {{verbs.data(), verbs.size()},
+                             {points.data(), points.size() * sizeof(float)},
+                             {weights.data(), weights.size() * sizeof(float)},
+                             {voff.data(), voff.size() * sizeof(int64_t)},
+                             {poff.data(), poff.size() * sizeof(int64_t)},
+                             {woff.data(), woff.size() * sizeof(int64_t)},
+                             {xf.data(), xf.size() * sizeof(Xform)},
+                             {flags.data(), flags.size()},
+                             {widths.data(), widths.size() * sizeof(float)}};
+    void* dv[9];
+    int st = upload_packed(dp, pc, 9, dv, s);
+    if (st) return st;
+    SegInput si;
+    si.n_segs = (int64_t)m;
+    si.n_paths = (int64_t)m;
+    si.verbs = static_cast<const uint8_t*>(dv[0]);
+    si.points = static_cast<const float*>(dv[1]);
+    si.weights = static_cast<const float*>(dv[2]);
+    si.seg_verb_off = static_cast<const int64_t*>(dv[3]);
+    si.seg_point_off = static_cast<const int64_t*>(dv[4]);
+    si.seg_weight_off = static_cast<const int64_t*>(dv[5]);
+    si.seg_xf = static_cast<const Xform*>(dv[6]);
+    si.n_verbs = (int64_t)verbs.size();
+    si.n_points = (int64_t)points.size() / 2;
+    si.n_weights = (int64_t)weights.size();
+    OutputDesc od;
+    int64_t tcap = std::max<int64_t>((int64_t)(dp->tile_alpha.bytes / 64), 64 * (int64_t)m);
+    int64_t scap = std::max<int64_t>((int64_t)(dp->span_w.bytes / 2), 8 * (int64_t)m);
+    const bool dense = ctx->c.dense;
+    ctx->c.dense = false;  // the ordered kernels: bit-exact, as a single finish is
+    for (int attempt = 0; attempt < 2; attempt++) {
+        hipError_t e;
+        if ((e = dp->tile_xy.ensure(tcap * 4)) != hipSuccess || (e = dp->tile_alpha.ensure(tcap * 64)) != hipSuccess ||
+            (e = dp->span_xy.ensure(std::max<int64_t>(scap, 1) * 4)) != hipSuccess ||
+            (e = dp->span_w.ensure(std::max<int64_t>(scap, 1) * 2)) != hipSuccess ||
+            (e = dp->pts.ensure(m * 8)) != hipSuccess || (e = dp->ptc.ensure(m * 4)) != hipSuccess ||
+            (e = dp->pss.ensure(m * 8)) != hipSuccess || (e = dp->psc.ensure(m * 4)) != hipSuccess) {
+            st = map_hip_error(e);
+            break;
+        }
+        od.tile_xy = dp->tile_xy.as<int16_t>();
+        od.tile_alpha = dp->tile_alpha.as<uint8_t>();
+        od.tile_capacity = (int64_t)(dp->tile_alpha.bytes / 64);
+        od.span_xy = dp->span_xy.as<int16_t>();
+        od.span_w = dp->span_w.as<uint16_t>();
+        od.span_capacity = (int64_t)(dp->span_w.bytes / 2);
+        od.path_tile_start = dp->pts.as<int64_t>();
+        od.path_tile_count = dp->ptc.as<int32_t>();
+        od.path_span_start = dp->pss.as<int64_t>();
+        od.path_span_count = dp->psc.as<int32_t>();
+        st = any_stroke ? run_stroke(&ctx->c, si, static_cast<const float*>(dv[8]),
+                                     static_cast<const uint8_t*>(dv[7]), od, s, true)
+                        : run_batched(&ctx->c, si, od, s, ctx->c.batched_waves_per_cu, ctx->c.batched_gran,
+                                      ctx->c.batched_tc, false);
+        if (st != OCHRE_ERR_CAPACITY) break;
+        tcap = od.n_tiles;
+        scap = od.n_spans;
+    }
+    ctx->c.dense = dense;
+    if (st) return st;
+    std::vector<int16_t> txy((size_t)od.n_tiles * 2), sxy((size_t)od.n_spans * 2);
+    std::vector<uint8_t> ta((size_t)od.n_tiles * 64);
+    std::vector<uint16_t> sw((size_t)od.n_spans);
+    std::vector<int64_t> ts(m), ss(m);
+    std::vector<int32_t> tc(m), sc(m);
+    hipError_t e = hipSuccess;
+    auto get = [&](void* dst, const void* src, size_t bytes) {
+        if (e == hipSuccess && bytes) e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s);
+    };
+    get(txy.data(), od.tile_xy, txy.size() * 2);
+    get(ta.data(), od.tile_alpha, ta.size());
+    get(sxy.data(), od.span_xy, sxy.size() * 2);
+    get(sw.data(), od.span_w, sw.size() * 2);
+    get(ts.data(), od.path_tile_start, m * 8);
+    get(tc.data(), od.path_tile_count, m * 4);
+    get(ss.data(), od.path_span_start, m * 8);
+    get(sc.data(), od.path_span_count, m * 4);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return map_hip_error(e);
+    for (size_t k = 0; k < m; k++)
+        replay(builder, txy.data() + 2 * ts[k], ta.data() + 64 * ts[k], tc[k], sxy.data() + 2 * ss[k],
+               sw.data() + ss[k], sc[k]);
+    return OCHRE_OK;
+}
+
+int ochre_rasterizer_finish_many(ochre_rasterizer* const* rs, size_t n, const ochre_tile_builder* builder) {
+    if (n && !rs) return OCHRE_ERR_INVALID_ARG;
+    int first_error = OCHRE_OK;
+    size_t i = 0;
+    while (i < n) {
+        size_t j = i;
+        while (j < n && rs[j] && single_draw(rs[j]) && rs[j]->ctx == rs[i]->ctx) j++;
+        int st;
+        if (j - i >= 2) {
+            st = finish_draws(rs + i, j - i, builder);
+            for (size_t k = i; k < j; k++) delete rs[k];
+            i = j;
+        } else {
+            st = rs[i] ? ochre_rasterizer_finish(rs[i], builder) : OCHRE_ERR_INVALID_ARG;
+            i++;
+        }
+        if (st && first_error == OCHRE_OK) first_error = st;
+    }
+    return first_error;
 }
 
 }  // extern "C"

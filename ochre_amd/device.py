@@ -375,6 +375,18 @@ def autotune(ctx: Context, db: DeviceBatch, out: DeviceOutput, reps=3, cache_key
     return best, allm
 
 
+def autotune_cached(ctx: Context, db: DeviceBatch, out: DeviceOutput, reps=3, cache_path=None, stream=None):
+    """The library's cached autotuner: applies the stored parameters for this
+    device and batch signature, or tunes and stores them (appending to
+    `cache_path` when given).  Returns True when it tuned."""
+    tuned = ctypes.c_int()
+    cb, co = db.c(), out.c()
+    path = cache_path.encode() if cache_path else None
+    _lib.check(ctx.lib.ochre_autotune_batched_cached(ctx.h, ctypes.byref(cb), ctypes.byref(co), _stream_ptr(stream),
+                                                     reps, path, ctypes.byref(tuned)), "ochre_autotune_batched_cached")
+    return bool(tuned.value)
+
+
 def coverage_adjoint(ctx: Context, db: DeviceBatch, target: DeviceOutput, stream=None):
     """Loss per path and gradients w.r.t. the path-space points and the
     transforms (torch tensors on the device)."""

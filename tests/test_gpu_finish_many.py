@@ -1,0 +1,86 @@
+"""The drop-in Rasterizer's finish paths against the oracle Rasterizer, event
+for event: a single fill (one upload, one launch, one read-back), and
+finish_many over a sequence mixing single fills, single strokes, empty
+rasterizers and one with several fills in the middle (the demo's loop,
+examples/svg.rs:140-156, finished into one builder)."""
+import numpy as np
+import pytest
+
+import oracle_ref
+from ochre_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _same(events, ref, what):
+    assert len(events) == len(ref), (what, len(events), len(ref))
+    for a, b in zip(events, ref):
+        assert a[:3] == b[:3], (what, a[:3], b[:3])
+        if a[0] == "tile":
+            assert np.array_equal(a[3], b[3]), what
+        else:
+            assert a[3] == b[3], what
+
+
+def _oracle_events(kinds, batch):
+    ref = []
+    for k, (kind, i) in enumerate(kinds):
+        o = oracle_ref.Rasterizer()
+        if kind == "fill":
+            v, p, w, xf = batch.path(i)
+            o.fill(v, p, w, xf)
+        elif kind == "stroke":
+            v, p, w, xf = batch.path(i)
+            o.stroke(v, p, float(batch.widths[i]), w, xf)
+        elif kind == "two":
+            for j in (i, i + 1):
+                v, p, w, xf = batch.path(j)
+                o.fill(v, p, w, xf)
+        ref += o.finish()
+    return ref
+
+
+def _make(kinds, batch):
+    from ochre_amd import Rasterizer
+    rs = []
+    for kind, i in kinds:
+        r = Rasterizer()
+        if kind == "fill":
+            v, p, w, xf = batch.path(i)
+            r.fill_arrays(v, p, w, xf)
+        elif kind == "stroke":
+            v, p, w, xf = batch.path(i)
+            r.stroke_arrays(v, p, float(batch.widths[i]), w, xf)
+        elif kind == "two":
+            for j in (i, i + 1):
+                v, p, w, xf = batch.path(j)
+                r.fill_arrays(v, p, w, xf)
+        rs.append(r)
+    return rs
+
+
+def test_single_fills_one_by_one(gpu_ctx):
+    from ochre_amd import EventRecorder
+    b = W.concat([W.readme_path(), W.glyph_run(40, seed=51), W.random_paths(40, 52)])
+    kinds = [("fill", i) for i in range(b.n_paths)]
+    events = []
+    for r in _make(kinds, b):
+        rec = EventRecorder()
+        r.finish(rec)
+        events += rec.events
+    _same(events, _oracle_events(kinds, b), "single fills")
+
+
+def test_finish_many_matches_one_by_one_and_oracle(gpu_ctx):
+    from ochre_amd import EventRecorder, finish_many
+    b = W.with_widths(W.concat([W.glyph_run(60, seed=53), W.random_paths(40, 54)]), 0.5, 3.0, 9)
+    rng = np.random.default_rng(55)
+    kinds = []
+    for i in range(b.n_paths - 1):
+        r = rng.random()
+        kinds.append(("stroke", i) if r < 0.3 else ("empty", i) if r < 0.35 else ("fill", i))
+    kinds.insert(37, ("two", 10))  # several fills: finished on its own, in place
+    rec = EventRecorder()
+    finish_many(_make(kinds, b), rec)
+    print(f"[finish_many] {len(kinds)} rasterizers, {len(rec.events)} TileBuilder calls", flush=True)
+    _same(rec.events, _oracle_events(kinds, b), "finish_many")
