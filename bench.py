@@ -573,6 +573,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-secondary", action="store_true",
                     help="time only the headline workload (no star blobs, scenes, strokes, adjoint)")
+    ap.add_argument("--no-probes", action="store_true",
+                    help="skip the latency probes (single fills, the SVG draw list, per-path plain launches): "
+                         "a profiled run then traces the timed kernels only")
     ap.add_argument("--params", default=None, metavar="W,G,TC",
                     help="batched pipeline parameters (waves/CU, paths per pop, tiles per chunk) instead of autotuning")
     ap.add_argument("--dense", default=None, metavar="G,W",
@@ -726,12 +729,13 @@ def main():
         "kernels_ms": {m: r["kernels_ms"] for m, r in res.items()},
         "roofline": roofline(kname, res[head]["kernel_ms"], ms, alg_k, b_in + b_out, batch.name),
     }
-    line["rasterizer_api_latency"] = api_latency()
-    try:
-        line["svg_demo_latency"] = svg_demo_latency()
-    except Exception as e:  # report the bench line without it
-        log(f"svg demo latency probe skipped: {e}")
-    if not stroking:
+    if not args.no_probes:
+        line["rasterizer_api_latency"] = api_latency()
+        try:
+            line["svg_demo_latency"] = svg_demo_latency()
+        except Exception as e:  # report the bench line without it
+            log(f"svg demo latency probe skipped: {e}")
+    if not stroking and not args.no_probes:
         try:
             pp = plain_per_path(ctx, batch)
             line["plain_per_path_launch"] = pp

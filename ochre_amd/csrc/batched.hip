@@ -310,6 +310,25 @@ __device__ void fused_path(const FusedIn& in, const FusedOut& out, FusedCtl* ctl
             c.start = g > pb ? fload(in, g - 1, X) : mk(0.0f, 0.0f);
             c.first = mv_excl >= 0 ? fload(in, mv_excl, X) : mk(0.0f, 0.0f);
         }
+        // A certain overflow of the line list, from the curves' steps alone
+        // (t advances by at most dt + 1e-7 per step of the flatten loop): the
+        // path is handed on before any of its curves is evaluated here.
+        uint32_t lower = 0;
+        if (valid && !lane_bad && (verb == QUAD || verb == CUBIC)) {
+            const float dt = verb == QUAD ? quad_dt(c.start, c.pt[0], c.pt[1])
+                                          : cubic_dt(c.start, c.pt[0], c.pt[1], c.pt[2]);
+            const float est = 0.99f / (dt + 1e-7f);
+            if (est > 0.0f) lower = est >= (float)F_LCAP ? (uint32_t)F_LCAP + 1u : (uint32_t)est;
+        }
+        const uint32_t lower_tot = __shfl(wave_incl_scan(lower, OpAdd()), WAVE - 1, WAVE);
+        if (L + lower_tot > (uint32_t)F_LCAP) {
+            if (ADJ) {
+                if (lane == 0) atomicOr(&ctl->error, (uint32_t)ERR_CAPACITY);
+            } else if (lane == 0) {
+                deferred[atomicAdd(&ctl->deferred, 1u)] = (uint32_t)p;
+            }
+            return;
+        }
         uint32_t n = 0;
         if (valid && !lane_bad) {
             bool ok = command_lines(c, [&](V2 a, V2 b) {
