@@ -287,9 +287,13 @@ int ochre_context_set_pass_limits(ochre_context* ctx, int64_t max_lines, int64_t
  * f32 loss per path, dL/d(points) (path space, pairs, same layout as the
  * input points) and dL/d(transform) (m00 m01 m10 m11 ox oy per path).  The
  * target is a render in the ochre_batch_output layout (tile_coverage set).
- * Paths must fit one wave (<= 32 x 32 tile grid, <= 256 tiles, <= 1024
- * commands, <= 4096 points, <= 8192 lines): OCHRE_ERR_TOO_LARGE otherwise.
- * No reference counterpart (the reference has no derivatives). */
+ * A path within one wave's limits (a 32 x 32 tile grid, 256 tiles, 1024
+ * commands, 4096 points, 8192 lines) is differentiated by one wave; a larger
+ * one after the batch, alone on the device, with its lines recomputed from the
+ * commands and its tile grid in device memory.  OCHRE_ERR_TOO_LARGE only for a
+ * path over 2^24 lines or a grid over 2^21 tiles, or with coordinates beyond
+ * the 16-bit pixel range.  No reference counterpart (the reference has no
+ * derivatives). */
 typedef struct ochre_coverage_target {
     const int64_t* tile_start;
     const int32_t* tile_count;

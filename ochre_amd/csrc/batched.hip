@@ -322,11 +322,7 @@ __device__ void fused_path(const FusedIn& in, const FusedOut& out, FusedCtl* ctl
         }
         const uint32_t lower_tot = __shfl(wave_incl_scan(lower, OpAdd()), WAVE - 1, WAVE);
         if (L + lower_tot > (uint32_t)F_LCAP) {
-            if (ADJ) {
-                if (lane == 0) atomicOr(&ctl->error, (uint32_t)ERR_CAPACITY);
-            } else if (lane == 0) {
-                deferred[atomicAdd(&ctl->deferred, 1u)] = (uint32_t)p;
-            }
+            if (lane == 0) deferred[atomicAdd(&ctl->deferred, 1u)] = (uint32_t)p;  // (the adjoint's large path)
             return;
         }
         uint32_t n = 0;
@@ -384,11 +380,7 @@ __device__ void fused_path(const FusedIn& in, const FusedOut& out, FusedCtl* ctl
                       (int32_t)floorf(bmaxy), &tx0, &ty0, &GW, &GH);
     if (overflow || full || GW > F_GRID || GH > F_GRID ||
         (ADJ && (ve - vb > A_VCAP || pe - pb > A_PCAP))) {
-        if (ADJ) {
-            if (lane == 0) atomicOr(&ctl->error, (uint32_t)ERR_CAPACITY);
-        } else if (lane == 0) {
-            deferred[atomicAdd(&ctl->deferred, 1u)] = (uint32_t)p;
-        }
+        if (lane == 0) deferred[atomicAdd(&ctl->deferred, 1u)] = (uint32_t)p;  // (the adjoint's large path)
         return;
     }
 
@@ -504,7 +496,7 @@ __device__ void fused_path(const FusedIn& in, const FusedOut& out, FusedCtl* ctl
     const uint32_t NS = __shfl(rsp_incl, WAVE - 1, WAVE);
     if (ADJ) {
         if (NT > (uint32_t)A_TCAP || NS > (uint32_t)A_SCAP) {
-            if (lane == 0) atomicOr(&ctl->error, (uint32_t)ERR_CAPACITY);
+            if (lane == 0) deferred[atomicAdd(&ctl->deferred, 1u)] = (uint32_t)p;  // the large path
             return;
         }
         if (row_lane && row_spans) {
@@ -795,6 +787,7 @@ __global__ __launch_bounds__(WAVE) void k_fused(FusedIn in, FusedOut out, FusedC
 }
 
 #include "adjoint.inc"
+#include "adjoint_large.inc"
 #include "dense.inc"
 
 // ---------------------------------------------- deferred paths -> plain
@@ -829,6 +822,7 @@ __global__ void k_scatter_ranges(const uint32_t* ids, uint32_t n, const int64_t*
 
 struct BatchedState {
     DBuf scratch, ascratch, ctl, deferred, dense_ids, dense_ids2, dense_ids3;
+    DBuf large_cmd, large_grid;  // the large-path adjoint's scratch
     DBuf g_verbs, g_points, g_weights, g_voff, g_poff, g_woff, g_xf, g_ts, g_tc, g_ss, g_sc;
     int n_waves = 0;
     uint32_t gran = 1;
@@ -1101,6 +1095,137 @@ int run_batched(Context* ctx, const SegInput& in, OutputDesc& out, hipStream_t s
     return st;
 }
 
+// The large-path adjoint of path p (one path over the whole device; see
+// adjoint_large.inc).  `hp`: 2 KiB of pinned host memory for the read-backs.
+static int run_adjoint_large(const FusedIn& fi, const AdjArgs& a, int64_t p, hipStream_t s, BatchedState* B,
+                             uint8_t* hp) {
+    int64_t* off = reinterpret_cast<int64_t*>(hp);
+    HIPCHK_RET(hipMemcpyAsync(off, fi.verb_off + p, 16, hipMemcpyDeviceToHost, s));
+    HIPCHK_RET(hipMemcpyAsync(off + 2, fi.point_off + p, 16, hipMemcpyDeviceToHost, s));
+    HIPCHK_RET(hipStreamSynchronize(s));
+    const int64_t nv = off[1] - off[0], np = off[3] - off[2];
+    if (nv < 0 || np < 0) return -1;
+    // command scratch: cg, cmv, cw | cl | pcmd | slot | box, err, nt | acc
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    size_t o = 0;
+    const size_t o_cg = o;
+    o = al(o + (size_t)nv * 8);
+    const size_t o_cmv = o;
+    o = al(o + (size_t)nv * 8);
+    const size_t o_cw = o;
+    o = al(o + (size_t)nv * 8);
+    const size_t o_cl = o;
+    o = al(o + (size_t)(nv + 1) * 4);
+    const size_t o_pcmd = o;
+    o = al(o + (size_t)np * 4);
+    const size_t o_slot = o;
+    o = al(o + (size_t)nv * 48);
+    const size_t o_ctl = o;  // box[4], err, nt
+    o = al(o + 32);
+    const size_t o_acc = o;
+    o = al(o + 8 * 8);
+    HIPCHK_RET(B->large_cmd.ensure(o));
+    uint8_t* c = B->large_cmd.as<uint8_t>();
+    LargeAdj A;
+    memset(&A, 0, sizeof(A));
+    A.cg = reinterpret_cast<int64_t*>(c + o_cg);
+    A.cmv = reinterpret_cast<int64_t*>(c + o_cmv);
+    A.cw = reinterpret_cast<int64_t*>(c + o_cw);
+    A.cl = reinterpret_cast<uint32_t*>(c + o_cl);
+    A.pcmd = reinterpret_cast<uint32_t*>(c + o_pcmd);
+    A.slot = reinterpret_cast<float*>(c + o_slot);
+    A.box = reinterpret_cast<unsigned*>(c + o_ctl);
+    A.err = reinterpret_cast<uint32_t*>(c + o_ctl + 16);
+    A.nt = reinterpret_cast<uint32_t*>(c + o_ctl + 20);
+    A.acc = reinterpret_cast<double*>(c + o_acc);
+    HIPCHK_RET(hipMemsetAsync(A.box, 0xff, 8, s));        // min x, min y
+    HIPCHK_RET(hipMemsetAsync(A.box + 2, 0, 16, s));      // max x, max y, err, nt
+    HIPCHK_RET(hipMemsetAsync(A.acc, 0, 8 * 8, s));
+    const int T = 256;
+    const unsigned gv = (unsigned)std::max<int64_t>(1, (nv + T - 1) / T);
+    hipLaunchKernelGGL(k_al_cmds, dim3(1), dim3(1024), 0, s, fi, p, A);
+    hipLaunchKernelGGL(k_al_count, dim3(gv), dim3(T), 0, s, fi, p, A);
+    hipLaunchKernelGGL(k_al_scan, dim3(1), dim3(1024), 0, s, fi, p, A);
+    HIPCHK_RET(hipGetLastError());
+    uint32_t* hc = reinterpret_cast<uint32_t*>(hp + 64);
+    HIPCHK_RET(hipMemcpyAsync(hc, A.cl + nv, 4, hipMemcpyDeviceToHost, s));
+    HIPCHK_RET(hipMemcpyAsync(hc + 1, A.box, 24, hipMemcpyDeviceToHost, s));  // box[4], err, nt
+    HIPCHK_RET(hipStreamSynchronize(s));
+    const uint32_t e1 = hc[5];
+    if (e1 & ERR_INPUT) return -1;
+    if (e1 & ERR_FLATTEN) return -5;
+    if (e1 & ERR_CAPACITY) return -7;  // coordinates beyond the 16-bit pixel range
+    const int64_t L = hc[0];
+    if (L >= AL_MAX_LINES) return -7;
+    int32_t tx0 = 0, ty0 = 0, GW = 0, GH = 0;
+    if (L > 0)
+        grid_from_box(true, false, (int32_t)floorf(unord_f(hc[1])), (int32_t)floorf(unord_f(hc[2])),
+                      (int32_t)floorf(unord_f(hc[3])), (int32_t)floorf(unord_f(hc[4])), &tx0, &ty0, &GW, &GH);
+    const int64_t tiles = (int64_t)GW * GH;
+    if (tiles > AL_MAX_TILES) return -7;
+    const int32_t ow = (GW + 31) / 32;
+    // line and grid scratch
+    o = 0;
+    const size_t o_l32 = o;
+    o = al(o + (size_t)L * 16);
+    const size_t o_l64 = o;
+    o = al(o + (size_t)L * 32);
+    const size_t o_dl = o;
+    o = al(o + (size_t)L * 16);
+    const size_t o_occ = o;
+    o = al(o + (size_t)ow * GH * 4);
+    const size_t o_fa = o;
+    o = al(o + (size_t)tiles * 64 * 8);
+    const size_t o_fh = o;
+    o = al(o + (size_t)tiles * 64 * 8);
+    const size_t o_ts = o;
+    o = al(o + (size_t)tiles * 4);
+    const size_t o_span = o;
+    o = al(o + (size_t)tiles);
+    const size_t o_g = o;
+    o = al(o + (size_t)tiles * 64 * 4);
+    const size_t o_G = o;
+    o = al(o + (size_t)tiles * 64 * 4);
+    HIPCHK_RET(B->large_grid.ensure(std::max<size_t>(o, 256)));
+    uint8_t* gbuf = B->large_grid.as<uint8_t>();
+    A.l32 = reinterpret_cast<float4*>(gbuf + o_l32);
+    A.l64 = reinterpret_cast<double2*>(gbuf + o_l64);
+    A.dl = reinterpret_cast<float4*>(gbuf + o_dl);
+    A.occ = reinterpret_cast<uint32_t*>(gbuf + o_occ);
+    A.fa = reinterpret_cast<unsigned long long*>(gbuf + o_fa);
+    A.fh = reinterpret_cast<unsigned long long*>(gbuf + o_fh);
+    A.tsign = reinterpret_cast<int32_t*>(gbuf + o_ts);
+    A.span = gbuf + o_span;
+    A.g = reinterpret_cast<float*>(gbuf + o_g);
+    A.G = reinterpret_cast<float*>(gbuf + o_G);
+    A.tx0 = tx0;
+    A.ty0 = ty0;
+    A.GW = GW;
+    A.GH = GH;
+    A.ow = ow;
+    if (o_g > o_occ) HIPCHK_RET(hipMemsetAsync(gbuf + o_occ, 0, o_g - o_occ, s));  // occ, sums, crossings, spans
+    const unsigned gl = (unsigned)std::max<int64_t>(1, (L + T - 1) / T);
+    const unsigned gp = (unsigned)std::max<int64_t>(1, (np + T - 1) / T);
+    const unsigned gr = (unsigned)std::max<int64_t>(1, ((int64_t)GH * TILE + T - 1) / T);
+    hipLaunchKernelGGL(k_al_lines, dim3(gv), dim3(T), 0, s, fi, p, A);
+    if (L > 0) hipLaunchKernelGGL(k_al_fwd, dim3(gl), dim3(T), 0, s, L, A);
+    hipLaunchKernelGGL(k_al_spans, dim3(1), dim3(1024), 0, s, A);
+    if (GH > 0) hipLaunchKernelGGL(k_al_rows, dim3(gr), dim3(T), 0, s, fi, A, a, p);
+    // target tiles: one thread each (their count is the caller's, bounded by its capacity)
+    hipLaunchKernelGGL(k_al_target, dim3(4096), dim3(T), 0, s, fi, A, a, p);
+    if (L > 0) hipLaunchKernelGGL(k_al_bwd, dim3(gl), dim3(T), 0, s, L, A);
+    hipLaunchKernelGGL(k_al_cmd_bwd, dim3(gv), dim3(T), 0, s, fi, p, A);
+    if (np > 0) hipLaunchKernelGGL(k_al_gather, dim3(gp), dim3(T), 0, s, fi, p, A, a);
+    hipLaunchKernelGGL(k_al_final, dim3(1), dim3(1), 0, s, A, a, p);
+    HIPCHK_RET(hipGetLastError());
+    HIPCHK_RET(hipMemcpyAsync(hc, A.err, 4, hipMemcpyDeviceToHost, s));
+    HIPCHK_RET(hipStreamSynchronize(s));
+    if (hc[0] & ERR_INPUT) return -1;
+    if (hc[0] & ERR_FLATTEN) return -5;
+    if (hc[0] & ERR_BBOX) return -3;
+    return 0;
+}
+
 int run_adjoint(Context* ctx, const SegInput& in, const AdjTarget& tgt, float* loss, float* d_points, float* d_xf,
                 hipStream_t s, int waves_per_cu, int gran, int tiles_per_chunk) {
     if (!ctx->batched) ctx->batched = new BatchedState();
@@ -1146,12 +1271,14 @@ int run_adjoint(Context* ctx, const SegInput& in, const AdjTarget& tgt, float* l
         if (!ctx->ev[1]) HIPCHK_RET(hipEventCreate(&ctx->ev[1]));
         HIPCHK_RET(hipEventRecord(ctx->ev[0], s));
     }
+    HIPCHK_RET(B->deferred.ensure((size_t)NP * 4));
+    uint32_t* large = B->deferred.as<uint32_t>();
     if (tiles_per_chunk <= 16)
         hipLaunchKernelGGL(k_adjoint<16>, dim3(n_waves), dim3(WAVE), 0, s, fi, a, ctl, B->scratch.as<uint8_t>(),
-                           B->ascratch.as<uint8_t>(), g);
+                           B->ascratch.as<uint8_t>(), g, large);
     else
         hipLaunchKernelGGL(k_adjoint<32>, dim3(n_waves), dim3(WAVE), 0, s, fi, a, ctl, B->scratch.as<uint8_t>(),
-                           B->ascratch.as<uint8_t>(), g);
+                           B->ascratch.as<uint8_t>(), g, large);
     HIPCHK_RET(hipGetLastError());
     if (ctx->timing) HIPCHK_RET(hipEventRecord(ctx->ev[1], s));
     FusedCtl* hc = reinterpret_cast<FusedCtl*>(reinterpret_cast<uint8_t*>(ctx->host_pinned) + 1024);
@@ -1168,6 +1295,18 @@ int run_adjoint(Context* ctx, const SegInput& in, const AdjTarget& tgt, float* l
     if (hc->error & ERR_FLATTEN) return -5;
     if (hc->error & ERR_BBOX) return -3;
     if (hc->error & ERR_CAPACITY) return -7;
+    // the paths over one wave's limits, one at a time over the whole device
+    const uint32_t n_large = hc->deferred;
+    if (n_large) {
+        std::vector<uint32_t> ids(n_large);
+        HIPCHK_RET(hipMemcpy(ids.data(), large, (size_t)n_large * 4, hipMemcpyDeviceToHost));
+        std::sort(ids.begin(), ids.end());
+        uint8_t* hp = reinterpret_cast<uint8_t*>(ctx->host_pinned) + 2048;
+        for (uint32_t id : ids) {
+            const int st = run_adjoint_large(fi, a, (int64_t)id, s, B, hp);
+            if (st) return st;
+        }
+    }
     return 0;
 }
 

@@ -183,7 +183,7 @@ __device__ inline void walk_line(V2 last, V2 point, Sink& sink) {
 // steps they may be off by k * 2^-24, i.e. up to span^2 * 2^-24 pixels, plus
 // one pixel of floor().  Paths with huge or non-finite coordinates get the
 // whole i16 range (the DDA's coordinates wrap in i16).
-__device__ inline void grid_from_box(bool has_lines, bool full, int32_t minx, int32_t miny, int32_t maxx,
+__host__ __device__ inline void grid_from_box(bool has_lines, bool full, int32_t minx, int32_t miny, int32_t maxx,
                                      int32_t maxy, int32_t* tx0, int32_t* ty0, int32_t* w, int32_t* h) {
     *tx0 = *ty0 = *w = *h = 0;
     if (!has_lines) return;

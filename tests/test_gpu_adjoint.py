@@ -50,13 +50,48 @@ def _check_adjoint(ctx, b, tb, fd_samples, seed):
     """Loss and every gradient element of the device adjoint against the f64
     oracle adjoint, and `fd_samples` seeded parameters against central finite
     differences of the f64 forward."""
+    tgt = render_target(ctx, tb)
+    _check_adjoint_against(ctx, b, tgt, [host_target(tgt, i) for i in range(b.n_paths)], fd_samples, seed)
+
+
+def oracle_target(ctx, tb):
+    """A target render from the oracles: tiles with the f64 forward's coverage
+    and the f32 oracle's spans (the two tile sets are identical), as a device
+    output with per-path ranges, and the same per path on the host."""
+    import torch
+    from ochre_amd.device import DeviceOutput
+    ref = O.run_batch(tb)
+    txys, covs, hosts = [], [], []
+    for i in range(tb.n_paths):
+        v, p, w, xf = tb.path(i)
+        n = int(ref.path_tile_off[i + 1] - ref.path_tile_off[i])
+        st, _, _, _, (txy, cov) = O.path_loss64(v, p.astype(np.float64), w, xf.astype(np.float64), want_cov=True,
+                                                cov_cap=max(n, 1))
+        assert st == 0 and np.array_equal(txy, ref.tile_xy[ref.path_tile_off[i]:ref.path_tile_off[i + 1]])
+        s0, s1 = ref.path_span_off[i], ref.path_span_off[i + 1]
+        txys.append(txy)
+        covs.append(cov.astype(np.float32))
+        hosts.append(O.Target(txy, cov.astype(np.float32), ref.span_xy[s0:s1], ref.span_w[s0:s1]))
+    nt, ns = ref.tile_xy.shape[0], ref.span_xy.shape[0]
+    out = DeviceOutput(tb.n_paths, nt, ns, coverage=True)
+    out.tile_xy[:nt] = torch.from_numpy(np.concatenate(txys).astype(np.int16))
+    out.tile_cov[:nt] = torch.from_numpy(np.concatenate(covs))
+    out.span_xy[:ns] = torch.from_numpy(ref.span_xy.astype(np.int16))
+    out.span_w[:ns] = torch.from_numpy(ref.span_w.astype(np.uint16).view(np.int16))
+    out.path_tile_start[:] = torch.from_numpy(ref.path_tile_off[:-1].astype(np.int64))
+    out.path_tile_count[:] = torch.from_numpy(np.diff(ref.path_tile_off).astype(np.int32))
+    out.path_span_start[:] = torch.from_numpy(ref.path_span_off[:-1].astype(np.int64))
+    out.path_span_count[:] = torch.from_numpy(np.diff(ref.path_span_off).astype(np.int32))
+    out.n_tiles, out.n_spans = nt, ns
+    return out, hosts
+
+
+def _check_adjoint_against(ctx, b, tgt, targets, fd_samples, seed):
     import torch
     from ochre_amd.device import coverage_adjoint, upload
-    tgt = render_target(ctx, tb)
     loss, dp, dx = coverage_adjoint(ctx, upload(b), tgt)
     torch.cuda.synchronize()
     loss, dp, dx = loss.cpu().numpy(), dp.cpu().numpy(), dx.cpu().numpy()
-    targets = [host_target(tgt, i) for i in range(b.n_paths)]
     errs = []
     worst = (0.0, None)
     beyond = []  # elements off the f64 gradient by more than 1e-3
@@ -162,3 +197,16 @@ def test_adjoint_side_batch(gpu_ctx):
     """Another seed of the same generator (96 paths)."""
     b = W.adjoint_glyphs(n=96, seed=100)
     _check_adjoint(gpu_ctx, b, perturbed(b, 0.5, 901), fd_samples=32, seed=5)
+
+
+def test_adjoint_large_paths(gpu_ctx):
+    """Paths over one wave's limits (star blobs of 1,000-1,500 cubics, a few
+    thousand tiles each) go through the large-path adjoint: every element
+    against the f64 oracle adjoint and finite differences on seeded samples."""
+    b = W.star_blobs(n=3, seed=71, nmin=1e3, nmax=1.5e3, rmin=120.0, rmax=250.0)
+    b.name = "star_blobs(n=3, 1k-1.5k cubics)"
+    tb = perturbed(b, 0.5, 902)
+    tgt, hosts = oracle_target(gpu_ctx, tb)
+    print(f"[adjoint/large] {b.n_paths} blobs, {int(np.diff(b.verb_off).max())} verbs max, "
+          f"{tgt.n_tiles} target tiles", flush=True)
+    _check_adjoint_against(gpu_ctx, b, tgt, hosts, fd_samples=12, seed=6)
