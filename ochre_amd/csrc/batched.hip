@@ -789,6 +789,7 @@ __global__ __launch_bounds__(WAVE) void k_fused(FusedIn in, FusedOut out, FusedC
 #include "adjoint.inc"
 #include "adjoint_large.inc"
 #include "dense.inc"
+#include "single.inc"
 
 // ---------------------------------------------- deferred paths -> plain
 
@@ -1312,7 +1313,8 @@ int run_adjoint(Context* ctx, const SegInput& in, const AdjTarget& tgt, float* l
 
 // ------------------------------------- a few paths, no host round trip
 //
-// The ordered kernel over a small batch (a Rasterizer's single fill): the
+// The ordered kernel over a small batch (a Rasterizer's single fill; one path
+// takes k_single, a workgroup of sixteen waves): the
 // queue reset and one wave per path, launched directly, then an asynchronous
 // copy of the control block to `host_ctl` (pinned, SMALL_CTL_BYTES).  Nothing
 // synchronises; after the caller's synchronisation small_batch_result reads
@@ -1354,8 +1356,11 @@ int launch_small_batch(Context* ctx, const SegInput& in, const OutputDesc& out, 
     fo.psc = out.path_span_count;
     FusedCtl* ctl = B->ctl.as<FusedCtl>();
     hipLaunchKernelGGL(k_fused_reset, dim3(1), dim3(64), 0, s, ctl);
-    hipLaunchKernelGGL(k_fused<32>, dim3(n_waves), dim3(WAVE), 0, s, fi, fo, ctl, B->deferred.as<uint32_t>(),
-                       B->scratch.as<uint8_t>(), 1u);
+    if (NP == 1)  // one path: one workgroup of sixteen waves (latency)
+        hipLaunchKernelGGL(k_single, dim3(1), dim3(S_NT), 0, s, fi, fo, ctl);
+    else
+        hipLaunchKernelGGL(k_fused<32>, dim3(n_waves), dim3(WAVE), 0, s, fi, fo, ctl, B->deferred.as<uint32_t>(),
+                           B->scratch.as<uint8_t>(), 1u);
     HIPCHK_RET(hipGetLastError());
     HIPCHK_RET(hipMemcpyAsync(host_ctl, ctl, sizeof(FusedCtl), hipMemcpyDeviceToHost, s));
     return 0;
