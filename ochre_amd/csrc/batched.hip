@@ -1355,12 +1355,13 @@ int launch_small_batch(Context* ctx, const SegInput& in, const OutputDesc& out, 
     fo.pss = out.path_span_start;
     fo.psc = out.path_span_count;
     FusedCtl* ctl = B->ctl.as<FusedCtl>();
-    hipLaunchKernelGGL(k_fused_reset, dim3(1), dim3(64), 0, s, ctl);
-    if (NP == 1)  // one path: one workgroup of sixteen waves (latency)
+    if (NP == 1) {  // one path: one workgroup of sixteen waves (latency), which resets its counts itself
         hipLaunchKernelGGL(k_single, dim3(1), dim3(S_NT), 0, s, fi, fo, ctl);
-    else
+    } else {
+        hipLaunchKernelGGL(k_fused_reset, dim3(1), dim3(64), 0, s, ctl);
         hipLaunchKernelGGL(k_fused<32>, dim3(n_waves), dim3(WAVE), 0, s, fi, fo, ctl, B->deferred.as<uint32_t>(),
                            B->scratch.as<uint8_t>(), 1u);
+    }
     HIPCHK_RET(hipGetLastError());
     HIPCHK_RET(hipMemcpyAsync(host_ctl, ctl, sizeof(FusedCtl), hipMemcpyDeviceToHost, s));
     return 0;
