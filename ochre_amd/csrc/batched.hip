@@ -1311,28 +1311,22 @@ int run_adjoint(Context* ctx, const SegInput& in, const AdjTarget& tgt, float* l
     return 0;
 }
 
-// ------------------------------------- a few paths, no host round trip
+// ------------------------------------------- one path, no host round trip
 //
-// The ordered kernel over a small batch (a Rasterizer's single fill; one path
-// takes k_single, a workgroup of sixteen waves): the
-// queue reset and one wave per path, launched directly, then an asynchronous
-// copy of the control block to `host_ctl` (pinned, SMALL_CTL_BYTES).  Nothing
-// synchronises; after the caller's synchronisation small_batch_result reads
-// the counts.  A path over the kernel's limits is only counted as deferred
-// (the caller reruns the batch through run_batched, which handles it).
+// A Rasterizer's single fill: k_single (one workgroup of sixteen waves) resets
+// the control block itself, writes the output and the counts wherever `out`
+// and `ctl` (SMALL_CTL_BYTES) point, and nothing synchronises.  The drop-in
+// finish points both at pinned host memory mapped into the device's address
+// space, so that no copy follows the launch; after the caller's
+// synchronisation small_batch_result reads the counts.  A path over the
+// kernel's limits is only counted as deferred (the caller reruns it through
+// run_batched, which handles it).
 static_assert(sizeof(FusedCtl) == SMALL_CTL_BYTES, "control block size");
 
-int launch_small_batch(Context* ctx, const SegInput& in, const OutputDesc& out, hipStream_t s, void* host_ctl) {
-    if (!ctx->batched) ctx->batched = new BatchedState();
-    BatchedState* B = reinterpret_cast<BatchedState*>(ctx->batched);
-    const int64_t NP = in.n_paths;
-    if (NP <= 0) return 0;
-    const int n_waves = (int)std::min<int64_t>(NP, 1024);
-    HIPCHK_RET(B->scratch.ensure((size_t)n_waves * F_SCRATCH));
-    HIPCHK_RET(B->ctl.ensure(sizeof(FusedCtl)));
-    HIPCHK_RET(B->deferred.ensure((size_t)NP * 4));
+int launch_single_fill(const SegInput& in, const OutputDesc& out, void* ctl, hipStream_t s) {
+    if (in.n_paths != 1) return -1;
     FusedIn fi;
-    fi.n_paths = NP;
+    fi.n_paths = 1;
     fi.verbs = in.verbs;
     fi.verb_off = in.seg_verb_off;
     fi.points = in.points;
@@ -1354,16 +1348,8 @@ int launch_small_batch(Context* ctx, const SegInput& in, const OutputDesc& out, 
     fo.ptc = out.path_tile_count;
     fo.pss = out.path_span_start;
     fo.psc = out.path_span_count;
-    FusedCtl* ctl = B->ctl.as<FusedCtl>();
-    if (NP == 1) {  // one path: one workgroup of sixteen waves (latency), which resets its counts itself
-        hipLaunchKernelGGL(k_single, dim3(1), dim3(S_NT), 0, s, fi, fo, ctl);
-    } else {
-        hipLaunchKernelGGL(k_fused_reset, dim3(1), dim3(64), 0, s, ctl);
-        hipLaunchKernelGGL(k_fused<32>, dim3(n_waves), dim3(WAVE), 0, s, fi, fo, ctl, B->deferred.as<uint32_t>(),
-                           B->scratch.as<uint8_t>(), 1u);
-    }
+    hipLaunchKernelGGL(k_single, dim3(1), dim3(S_NT), 0, s, fi, fo, reinterpret_cast<FusedCtl*>(ctl));
     HIPCHK_RET(hipGetLastError());
-    HIPCHK_RET(hipMemcpyAsync(host_ctl, ctl, sizeof(FusedCtl), hipMemcpyDeviceToHost, s));
     return 0;
 }
 

@@ -659,10 +659,10 @@ struct DevPath {
     void* stage_host = nullptr;
     size_t stage_cap = 0;
     DBuf stage_dev;
-    // a single fill's output in one device arena, read back with one copy
-    // into a pinned one (whose tail also receives the kernel's control block)
-    DBuf arena;
+    // a single fill's output and the kernel's control block (at the tail), in
+    // pinned host memory that the kernel writes through its device mapping
     void* arena_host = nullptr;
+    void* arena_dev = nullptr;
     size_t arena_host_cap = 0;
     int64_t fast_tiles = 256, fast_spans = 64;  // the arena's capacities
     ~DevPath() {
@@ -897,26 +897,30 @@ static void replay(const ochre_tile_builder* builder, const int16_t* txy, const 
     }
 }
 
-// A Rasterizer holding one fill: one upload (already queued), two launches,
-// one read-back of the control block and of the whole output arena, one
-// synchronisation, then the replay.  Returns 1 when the fill needs the
-// general path (a path over the ordered kernel's limits, or more output than
-// the arena holds: its capacities grow for the next call).
-static int finish_one_fill(ochre_context* ctx, DevPath* dp, const SegInput& si, hipStream_t s,
-                           const ochre_tile_builder* builder) {
+// A Rasterizer holding one fill: one upload (already queued), one launch
+// whose output and counts land in mapped pinned host memory, one
+// synchronisation, then the replay.  Returns 1 when the fill needs the general
+// path (a path over k_single's limits, or more output than the arena holds:
+// its capacities grow for the next call).
+static int finish_one_fill(DevPath* dp, const SegInput& si, hipStream_t s, const ochre_tile_builder* builder) {
     const int64_t tc = dp->fast_tiles, sc = dp->fast_spans;
     const size_t o_txy = (size_t)tc * 64, o_sxy = o_txy + (size_t)tc * 4, o_sw = o_sxy + (size_t)sc * 4;
     const size_t o_rng = (o_sw + (size_t)sc * 2 + 255) & ~(size_t)255, total = o_rng + 256;
-    hipError_t e = dp->arena.ensure(total);
-    if (e != hipSuccess) return map_hip_error(e);
     if (dp->arena_host_cap < total + SMALL_CTL_BYTES) {
         if (dp->arena_host) (void)hipHostFree(dp->arena_host);
-        dp->arena_host = nullptr;
+        dp->arena_host = dp->arena_dev = nullptr;
         dp->arena_host_cap = 0;
-        if (hipHostMalloc(&dp->arena_host, total + SMALL_CTL_BYTES) != hipSuccess) return OCHRE_ERR_OUT_OF_MEMORY;
+        if (hipHostMalloc(&dp->arena_host, total + SMALL_CTL_BYTES, hipHostMallocMapped) != hipSuccess)
+            return OCHRE_ERR_OUT_OF_MEMORY;
+        hipError_t e = hipHostGetDevicePointer(&dp->arena_dev, dp->arena_host, 0);
+        if (e != hipSuccess) {
+            (void)hipHostFree(dp->arena_host);
+            dp->arena_host = dp->arena_dev = nullptr;
+            return map_hip_error(e);
+        }
         dp->arena_host_cap = total + SMALL_CTL_BYTES;
     }
-    uint8_t* a = dp->arena.as<uint8_t>();
+    uint8_t* a = static_cast<uint8_t*>(dp->arena_dev);
     OutputDesc od;
     od.tile_alpha = a;
     od.tile_xy = reinterpret_cast<int16_t*>(a + o_txy);
@@ -928,11 +932,11 @@ static int finish_one_fill(ochre_context* ctx, DevPath* dp, const SegInput& si, 
     od.path_tile_count = reinterpret_cast<int32_t*>(a + o_rng + 8);
     od.path_span_start = reinterpret_cast<int64_t*>(a + o_rng + 16);
     od.path_span_count = reinterpret_cast<int32_t*>(a + o_rng + 24);
-    uint8_t* h = static_cast<uint8_t*>(dp->arena_host);
-    int st = launch_small_batch(&ctx->c, si, od, s, h + total);
+    int st = launch_single_fill(si, od, a + total, s);
     if (st) return st;
-    if ((e = hipMemcpyAsync(h, a, total, hipMemcpyDeviceToHost, s)) != hipSuccess) return map_hip_error(e);
-    if ((e = hipStreamSynchronize(s)) != hipSuccess) return map_hip_error(e);
+    hipError_t e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return map_hip_error(e);
+    const uint8_t* h = static_cast<const uint8_t*>(dp->arena_host);
     const SmallBatchResult res = small_batch_result(h + total);
     if (res.status == OCHRE_ERR_CAPACITY || res.tiles > tc || res.spans > sc) {
         dp->fast_tiles = std::max<int64_t>(tc, res.tiles + res.tiles / 4 + 64);
@@ -1011,7 +1015,7 @@ int ochre_rasterizer_finish(ochre_rasterizer* r, const ochre_tile_builder* build
             SegInput s1 = si;
             s1.seg_path = nullptr;
             s1.path_seg_off = nullptr;
-            st = finish_one_fill(ctx, dp, s1, s, builder);
+            st = finish_one_fill(dp, s1, s, builder);
             if (st != 1) break;  // done (or an error); 1: the general path below
             st = 0;
         }

@@ -167,18 +167,20 @@ int run_plain(Context* ctx, const SegInput& in, OutputDesc& out, hipStream_t str
 // a captured HIP graph.
 int run_batched(Context* ctx, const SegInput& in, OutputDesc& out, hipStream_t stream, int waves_per_cu, int gran,
                 int tiles_per_chunk, bool use_graph);
-// The ordered kernel over a small batch with no host round trip: launches,
-// then copies the control block (SMALL_CTL_BYTES) to the pinned `host_ctl`
-// asynchronously; after synchronising, small_batch_result gives the counts,
-// the status of run_batched and how many paths were over the kernel's limits
-// (their output is not written: rerun through run_batched).
+// One path by k_single with no host round trip: the output goes where `out`
+// points and the control block (SMALL_CTL_BYTES, reset by the kernel) to
+// `ctl`, both device-visible (pinned host memory mapped into the device's
+// address space, for the drop-in finish); after synchronising,
+// small_batch_result gives the counts, the status of run_batched and whether
+// the path was over the kernel's limits (its output is not written: rerun
+// through run_batched).
 constexpr size_t SMALL_CTL_BYTES = 256;
 struct SmallBatchResult {
     int64_t tiles, spans;
     uint32_t deferred;
     int status;
 };
-int launch_small_batch(Context* ctx, const SegInput& in, const OutputDesc& out, hipStream_t stream, void* host_ctl);
+int launch_single_fill(const SegInput& in, const OutputDesc& out, void* ctl, hipStream_t stream);
 SmallBatchResult small_batch_result(const void* host_ctl);
 
 // The dense kernels alone (the 64-tile instance, then the 256-tile one over
