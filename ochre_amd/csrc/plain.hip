@@ -320,6 +320,13 @@ struct StoreRowBase {
     __device__ void operator()(int64_t i, uint32_t excl, uint32_t) const { g[i].row_base = excl; }
 };
 
+// In the unordered tile stage the run that starts a line has no record (its
+// replay starts where the line does): its entry's value is the line; the
+// value of a later run is its record's index with RT_REC set.  (Replaying
+// every run from its line's start instead of keeping records measured slower:
+// the extra steps cost more than the records' writes and gathers.)
+constexpr uint32_t RT_REC = 0x40000000u;
+
 struct CountSink {
     uint32_t runs = 0, entries = 0;
     __device__ void run(int16_t, int16_t, bool, const Dda&) {
@@ -334,6 +341,7 @@ struct CountSink {
 struct EmitSink {
     uint32_t line;
     uint32_t entry, run_idx;
+    bool unordered;     // the unordered tile stage's entries (see RT_FIRST_RUN)
     PathGrid g;
     uint32_t xbits;
     uint32_t* keys;
@@ -354,7 +362,12 @@ struct EmitSink {
         uint32_t k = 0;
         key_of(tx, ty, &k);
         keys[entry] = k;
-        vals[entry] = run_idx;
+        if (unordered && start) {
+            vals[entry] = line;
+            entry++;
+            return;
+        }
+        vals[entry] = run_idx | (unordered ? RT_REC : 0u);
         RunRec r;
         r.line = line | (start ? 0x80000000u : 0u);
         r.xy = (uint32_t)(uint16_t)d.x | ((uint32_t)(uint16_t)d.y << 16);
@@ -387,7 +400,9 @@ struct DdaArgs {
     Counters* cnt;
     uint64_t* blk_entries;     // count pass: the entries of each block's lines
     uint64_t* blk_cnt;         // count pass: each block's (runs << 32) | entries; emit pass: exclusive offsets
+    bool unordered;            // the unordered tile stage's entries (RT_REC)
 };
+
 
 // Line offsets: the count pass leaves each line's packed (runs, entries)
 // count and each block's sum; the blocks' sums are scanned, and the emit pass
@@ -405,7 +420,8 @@ __global__ __launch_bounds__(256) void k_dda(DdaArgs a) {
             CountSink s;
             s.cnt = a.cnt;
             walk_line(mk(L.x, L.y), mk(L.z, L.w), s);
-            c = ((uint64_t)s.runs << 32) | s.entries;
+            const uint32_t runs = (a.unordered && s.runs) ? s.runs - 1u : s.runs;  // no record for the first run
+            c = ((uint64_t)runs << 32) | s.entries;
             a.line_cnt[l] = c;
             e = s.entries;
         }
@@ -445,6 +461,7 @@ __global__ __launch_bounds__(256) void k_dda(DdaArgs a) {
         s.vals = a.vals;
         s.recs = a.recs;
         s.cnt = a.cnt;
+        s.unordered = a.unordered;
         walk_line(mk(L.x, L.y), mk(L.z, L.w), s);
     }
 }
@@ -1182,13 +1199,15 @@ __device__ inline void rt_load_keys(RtBatch& b, const RowTileArgs& a, uint32_t e
         b.vv[u] = e < ee ? a.vals[e] : 0x80000000u;
     }
 }
+// An entry's value: bit 31 a crossing (its sign in the low byte); else a
+// run record's index with RT_REC set, or the line whose first run it is.
 __device__ inline void rt_load_recs(RtBatch& b, const RowTileArgs& a) {
 #pragma unroll
     for (int u = 0; u < RT_U; u++) {
-        b.rr[u].line = 0;
+        b.rr[u].line = (b.vv[u] & (RT_REC - 1u)) | 0x80000000u;  // a line's first run: replayed from its start
         b.rr[u].xy = 0;
         b.rr[u].row_t1 = b.rr[u].col_t1 = 0.0f;
-        if (!(b.vv[u] >> 31)) b.rr[u] = a.recs[b.vv[u]];
+        if ((b.vv[u] >> 30) == 1u) b.rr[u] = a.recs[b.vv[u] & (RT_REC - 1u)];
     }
 }
 __device__ inline void rt_load_lines(RtBatch& b, const RowTileArgs& a) {
@@ -1553,6 +1572,7 @@ static int run_plain_pass(Context* ctx, const SegInput& in, OutputDesc& out, hip
     da.xbits = 0;
     da.keys = da.vals = nullptr;
     da.recs = nullptr;
+    da.unordered = unordered;
     const int64_t nblk = (int64_t)grid_for(std::max<int64_t>(L, 1), 256);
     HIPCHK(ctx->dda_blk.ensure((size_t)nblk * 16));
     da.blk_entries = ctx->dda_blk.as<uint64_t>();
