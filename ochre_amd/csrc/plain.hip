@@ -327,6 +327,39 @@ struct StoreRowBase {
 // the extra steps cost more than the records' writes and gathers.)
 constexpr uint32_t RT_REC = 0x40000000u;
 
+// walk_line's run and entry counts (CountSink) without the walk, for finite
+// lines within +-30000 of at most 4096 px per axis (rasterizer.rs:72-140).
+// Each step moves one pixel along x or y, taking the smaller of the two
+// crossing parameters, and the walk ends at the first parameter it takes
+// that equals 1.  Each axis' parameters form a chain of their own (c, min(c +
+// step, 1), ...) whatever the interleaving, so every value below 1 of both
+// chains is taken before the end: the walk's last pixel before the end is its
+// start moved by nx along x and ny along y (the chains' values below 1).  A
+// step changes one tile coordinate by at most one, so the runs are 1 + the
+// tile-x and tile-y changes, and the tile-row crossings are the tile-y changes
+// plus one when the end-point snap (:118-121) lands in another tile row.
+// Returns false for the lines it does not cover.
+__device__ inline bool line_counts(V2 last, V2 point, uint32_t& runs, uint32_t& entries) {
+    if (!line_finite(last, point) || !(fabsf(last.x) <= HUGE_COORD) || !(fabsf(last.y) <= HUGE_COORD) ||
+        !(fabsf(point.x) <= HUGE_COORD) || !(fabsf(point.y) <= HUGE_COORD) || fabsf(point.x - last.x) > 4096.0f ||
+        fabsf(point.y - last.y) > 4096.0f)
+        return false;
+    runs = entries = 0;
+    if (veq(point, last)) return true;  // rasterizer.rs:73
+    Dda d;
+    dda_init(d, last, point);
+    int32_t nx = 0, ny = 0;
+    for (float c = d.col_t1; c < 1.0f; c = fminf(c + d.x_step, 1.0f)) nx++;
+    for (float c = d.row_t1; c < 1.0f; c = fminf(c + d.y_step, 1.0f)) ny++;
+    const int32_t x0 = d.x, y0 = d.y;
+    const int32_t xl = x0 + (int32_t)d.x_dir * nx, yl = y0 + (int32_t)d.y_dir * ny;
+    const uint32_t ty_changes = (uint32_t)abs((yl >> 3) - (y0 >> 3));
+    runs = 1u + (uint32_t)abs((xl >> 3) - (x0 >> 3)) + ty_changes;
+    const int32_t ty_end = (int32_t)f2i16(floorf(point.y)) >> 3;
+    entries = runs + ty_changes + (ty_end != (yl >> 3) ? 1u : 0u);
+    return true;
+}
+
 struct CountSink {
     uint32_t runs = 0, entries = 0;
     __device__ void run(int16_t, int16_t, bool, const Dda&) {
@@ -419,7 +452,8 @@ __global__ __launch_bounds__(256) void k_dda(DdaArgs a) {
             float4 L = a.lines[l];
             CountSink s;
             s.cnt = a.cnt;
-            walk_line(mk(L.x, L.y), mk(L.z, L.w), s);
+            if (!line_counts(mk(L.x, L.y), mk(L.z, L.w), s.runs, s.entries))
+                walk_line(mk(L.x, L.y), mk(L.z, L.w), s);
             const uint32_t runs = (a.unordered && s.runs) ? s.runs - 1u : s.runs;  // no record for the first run
             c = ((uint64_t)runs << 32) | s.entries;
             a.line_cnt[l] = c;
