@@ -327,8 +327,8 @@ struct StoreRowBase {
 // the extra steps cost more than the records' writes and gathers.)
 constexpr uint32_t RT_REC = 0x40000000u;
 
-// walk_line's run and entry counts (CountSink) without the walk, for finite
-// lines within +-30000 of at most 4096 px per axis (rasterizer.rs:72-140).
+// walk_line's run and entry counts (CountSink) without the walk, for the lines
+// chain_walk_ok covers (rasterizer.rs:72-140).
 // Each step moves one pixel along x or y, taking the smaller of the two
 // crossing parameters, and the walk ends at the first parameter it takes
 // that equals 1.  Each axis' parameters form a chain of their own (c, min(c +
@@ -340,10 +340,7 @@ constexpr uint32_t RT_REC = 0x40000000u;
 // plus one when the end-point snap (:118-121) lands in another tile row.
 // Returns false for the lines it does not cover.
 __device__ inline bool line_counts(V2 last, V2 point, uint32_t& runs, uint32_t& entries) {
-    if (!line_finite(last, point) || !(fabsf(last.x) <= HUGE_COORD) || !(fabsf(last.y) <= HUGE_COORD) ||
-        !(fabsf(point.x) <= HUGE_COORD) || !(fabsf(point.y) <= HUGE_COORD) || fabsf(point.x - last.x) > 4096.0f ||
-        fabsf(point.y - last.y) > 4096.0f)
-        return false;
+    if (!chain_walk_ok(last, point)) return false;
     runs = entries = 0;
     if (veq(point, last)) return true;  // rasterizer.rs:73
     Dda d;

@@ -177,6 +177,66 @@ __device__ inline void walk_line(V2 last, V2 point, Sink& sink) {
 }
 
 
+// Lines that line_counts and line_tiles cover: finite, within +-30000, at
+// most 4096 px per axis (no i16 wrap, far below the walk's step limit).
+__device__ inline bool chain_walk_ok(V2 last, V2 point) {
+    return fabsf(last.x) <= HUGE_COORD && fabsf(last.y) <= HUGE_COORD && fabsf(point.x) <= HUGE_COORD &&
+           fabsf(point.y) <= HUGE_COORD && fabsf(point.x - last.x) <= 4096.0f && fabsf(point.y - last.y) <= 4096.0f;
+}
+
+// walk_line at tile granularity (rasterizer.rs:72-140): the tiles the walk
+// visits, in order (tile(tx, ty)), its tile-row crossings, the end-point
+// snap's included (cross(tx, ty, sign)), and its increments (`incs`), without
+// the per-pixel steps.  Each axis' crossing parameters form a chain of their
+// own whatever the interleaving (c, min(c + step, 1), ...); the step that
+// moves an axis into its next tile is k values along its chain (k from the
+// pixel's place in its tile, then 8), and of two such steps the one with the
+// smaller parameter comes first, the x step on a tie (dda_step takes y only
+// when row_t1 < col_t1).  Every value below 1 is taken before the walk ends,
+// so the increments are the chains' values below 1, plus the last one.  Only
+// for the lines chain_walk_ok covers; returns false, visiting nothing, for
+// the others.
+template <class Tile, class Cross>
+__device__ inline bool line_tiles(V2 last, V2 point, uint32_t& incs, Tile&& tile, Cross&& cross) {
+    incs = 0;
+    if (!chain_walk_ok(last, point)) return false;
+    if (veq(point, last)) return true;  // rasterizer.rs:73
+    Dda d;
+    dda_init(d, last, point);
+    int32_t tx = (int32_t)d.x >> 3, ty = (int32_t)d.y >> 3;
+    tile(tx, ty);
+    auto first_k = [](int32_t p, int dir) { return dir > 0 ? 8 - (p & 7) : (p & 7) + 1; };
+    auto advance = [](float c, float step, int k) {
+        for (int m = 1; m < k; m++) c = fminf(c + step, 1.0f);
+        return c;
+    };
+    float ex = d.x_dir ? advance(d.col_t1, d.x_step, first_k(d.x, d.x_dir)) : 1.0f;
+    float ey = d.y_dir ? advance(d.row_t1, d.y_step, first_k(d.y, d.y_dir)) : 1.0f;
+    for (;;) {
+        const bool xe = ex < 1.0f, ye = ey < 1.0f;
+        if (xe && (!ye || ex <= ey)) {
+            tx += d.x_dir;
+            tile(tx, ty);
+            ex = advance(fminf(ex + d.x_step, 1.0f), d.x_step, 8);
+        } else if (ye) {
+            const int32_t nty = ty + d.y_dir;
+            cross(tx, nty < ty ? nty : ty, (int)d.y_dir);
+            ty = nty;
+            tile(tx, ty);
+            ey = advance(fminf(ey + d.y_step, 1.0f), d.y_step, 8);
+        } else {
+            break;
+        }
+    }
+    uint32_t nx = 0, ny = 0;
+    for (float c = d.col_t1; c < 1.0f; c = fminf(c + d.x_step, 1.0f)) nx++;
+    for (float c = d.row_t1; c < 1.0f; c = fminf(c + d.y_step, 1.0f)) ny++;
+    incs = nx + ny + 1u;
+    const int32_t ty_end = (int32_t)f2i16(floorf(point.y)) >> 3;  // the snap (rasterizer.rs:118-131)
+    if (ty_end != ty) cross((int32_t)f2i16(floorf(point.x)) >> 3, ty_end < ty ? ty_end : ty, (int)(ty_end - ty));
+    return true;
+}
+
 // Tile grid of a path from the bounding box (pixel floors) of all its line
 // end points, widened by the DDA's worst-case overshoot: the crossing
 // parameters are summed sequentially (rasterizer.rs:110,114), so after k
