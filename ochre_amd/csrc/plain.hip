@@ -1151,6 +1151,7 @@ constexpr int RT_NT = RT_NW * WAVE;
 constexpr int RT_ROWS = TILE / RT_NW;  // pixel rows whose carry each wave takes
 static_assert(RT_ROWS * RT_NW == TILE && RT_ROWS <= 2, "the carry takes one or two pixel rows per wave");
 constexpr int RT_U = 2;  // entries per thread whose loads are issued together
+constexpr int RT_ROW_CHUNK = 16;  // consecutive tile rows on one XCD
 
 // The window's per-pixel sums are 64-bit fixed point (units of 2^-39): LDS
 // integer atomics cost a fraction of float ones, and the sums of the
@@ -1286,7 +1287,18 @@ __global__ __launch_bounds__(RT_NT) void k_row_tiles(RowTileArgs a) {
         }
         __syncthreads();
     }
-    for (int64_t row = blockIdx.x; row < a.n_rows; row += gridDim.x) {
+    // Neighbouring tile rows gather the same lines and run records: runs of
+    // RT_ROW_CHUNK consecutive rows go to the workgroups of one XCD
+    // (workgroups are dealt round-robin over the eight XCDs by id), so that
+    // the second gather hits that XCD's L2; the runs themselves go round-robin
+    // over the XCDs, which keeps their loads even.
+    const int64_t grid_n = gridDim.x;
+    int64_t first_row = blockIdx.x;
+    if (grid_n % (8 * RT_ROW_CHUNK) == 0) {
+        const int64_t j = blockIdx.x / 8u;  // the workgroup's place among its XCD's
+        first_row = ((j / RT_ROW_CHUNK) * 8 + (int64_t)(blockIdx.x % 8u)) * RT_ROW_CHUNK + j % RT_ROW_CHUNK;
+    }
+    for (int64_t row = first_row; row < a.n_rows; row += grid_n) {
         const PathGrid G = a.grid[a.row_path[row]];
         const int32_t r = (int32_t)(row - (int64_t)G.row_base);
         if (r >= G.h) continue;  // the empty-path row
@@ -1716,8 +1728,9 @@ static int run_plain_pass(Context* ctx, const SegInput& in, OutputDesc& out, hip
         ra.span_xy = out.span_xy;
         ra.span_w = out.span_w;
         ra.cnt = cnt;
-        const uint32_t row_blocks = (uint32_t)std::min<int64_t>(ROWS, 1 << 16);
-        hipLaunchKernelGGL(k_row_tiles<false>, dim3(row_blocks), dim3(RT_NT), 0, s, ra);
+        int64_t row_blocks = std::min<int64_t>(ROWS, 1 << 16);
+        if (row_blocks >= 8 * RT_ROW_CHUNK) row_blocks -= row_blocks % (8 * RT_ROW_CHUNK);  // rows dealt by XCD
+        hipLaunchKernelGGL(k_row_tiles<false>, dim3((uint32_t)row_blocks), dim3(RT_NT), 0, s, ra);
         HIPCHK(launch_scan<uint64_t>(ctx, ROWS, LoadU64{ra.row_cnt}, StoreU64{ctx->row_off.as<uint64_t>()}, OpAdd(),
                                      0ull, &tot[4], s));
         HIPCHK(hipGetLastError());
@@ -1750,7 +1763,7 @@ static int run_plain_pass(Context* ctx, const SegInput& in, OutputDesc& out, hip
             if (!ctx->ev[1]) HIPCHK(hipEventCreate(&ctx->ev[1]));
             HIPCHK(hipEventRecord(ctx->ev[0], s));
         }
-        hipLaunchKernelGGL(k_row_tiles<true>, dim3(row_blocks), dim3(RT_NT), 0, s, ra);
+        hipLaunchKernelGGL(k_row_tiles<true>, dim3((uint32_t)row_blocks), dim3(RT_NT), 0, s, ra);
         if (ctx->timing) HIPCHK(hipEventRecord(ctx->ev[1], s));
         hipLaunchKernelGGL(k_path_out, dim3(grid_for(NP, 256)), dim3(256), 0, s, grid, NP, ra.row_off,
                            (const uint64_t*)&tot[4], out.path_tile_start, out.path_tile_count, out.path_span_start,
