@@ -152,8 +152,9 @@ typedef struct ochre_batch_output {
     int32_t* path_span_count; /* [n_paths] */
     int64_t n_tiles;         /* out */
     int64_t n_spans;         /* out */
-    float* tile_coverage;    /* optional, batched pipeline only: [tile_capacity][64] pre-quantisation
-                                coverage min(|accum + area|, 1) (rasterizer.rs:235 before * 256) */
+    float* tile_coverage;    /* optional: [tile_capacity][64] pre-quantisation coverage
+                                min(|accum + area|, 1) (rasterizer.rs:235 before * 256), written for
+                                every tile (handed-on paths included) when not NULL */
 } ochre_batch_output;
 
 typedef struct ochre_context ochre_context;

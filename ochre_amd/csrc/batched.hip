@@ -1067,6 +1067,7 @@ int run_batched(Context* ctx, const SegInput& in, OutputDesc& out, hipStream_t s
         const bool room = Tf <= out.tile_capacity && Sf <= out.span_capacity;
         od.tile_xy = out.tile_xy + (room ? 2 * Tf : 0);
         od.tile_alpha = out.tile_alpha + (room ? 64 * Tf : 0);
+        od.tile_cov = out.tile_cov ? out.tile_cov + (room ? 64 * Tf : 0) : nullptr;
         od.tile_capacity = room ? out.tile_capacity - Tf : 0;
         od.span_xy = out.span_xy + (room ? 2 * Sf : 0);
         od.span_w = out.span_w + (room ? Sf : 0);

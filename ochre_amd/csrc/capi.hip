@@ -127,6 +127,7 @@ int ochre_rasterize_plain(ochre_context* ctx, const ochre_path_batch* in, ochre_
     OutputDesc od;
     od.tile_xy = out->tile_xy;
     od.tile_alpha = out->tile_alpha;
+    od.tile_cov = out->tile_coverage;
     od.tile_capacity = out->tile_capacity;
     od.span_xy = out->span_xy;
     od.span_w = out->span_width;

@@ -640,6 +640,7 @@ struct TileArgs {
     const uint64_t* row_off;  // emit pass input: (tiles << 32) | spans exclusive offsets
     int16_t* tile_xy;
     uint8_t* tile_alpha;
+    float* tile_cov;          // optional pre-quantisation coverage, 64 per tile
     int16_t* span_xy;
     uint16_t* span_w;
     Counters* cnt;
@@ -790,12 +791,31 @@ constexpr int TR_MSTR = 64;        // bytes per run map row
 
 __device__ inline uint32_t rl(uint32_t v, int j) { return (uint32_t)__builtin_amdgcn_readlane((int)v, j); }
 
+// The first tile row of a workgroup of a row kernel (rows first_row +
+// k * gridDim.x).  Neighbouring tile rows gather the same lines and run
+// records: runs of ROW_CHUNK consecutive rows go to the workgroups of one XCD
+// (workgroups are dealt round-robin over the eight XCDs by id), so that the
+// second gather hits that XCD's L2; the runs themselves go round-robin over
+// the XCDs, which keeps their loads even.  Grids that are a multiple of
+// 8 * ROW_CHUNK only (row_grid); others take rows in id order.
+constexpr int ROW_CHUNK = 16;
+__device__ inline int64_t xcd_first_row() {
+    if (gridDim.x % (8u * ROW_CHUNK) != 0) return blockIdx.x;
+    const int64_t j = blockIdx.x / 8u;  // the workgroup's place among its XCD's
+    return ((j / ROW_CHUNK) * 8 + (int64_t)(blockIdx.x % 8u)) * ROW_CHUNK + j % ROW_CHUNK;
+}
+static inline uint32_t row_grid(int64_t rows, int64_t cap) {
+    int64_t g = std::min<int64_t>(rows, cap);
+    if (g >= 8 * ROW_CHUNK) g -= g % (8 * ROW_CHUNK);
+    return (uint32_t)g;
+}
+
 __global__ __launch_bounds__(WAVE) void k_tile_rows(TileArgs a) {
     __shared__ float2 inc[TR_BLOCK * TR_SSTR];
     __shared__ __align__(16) uint8_t smap[TR_BLOCK * TR_MSTR];  // run j, pixel p -> step + 1 (0: not visited)
     const int lane = threadIdx.x;
     const uint32_t xmask = (1u << a.xbits) - 1;
-    for (int64_t row = blockIdx.x; row < a.n_rows; row += gridDim.x) {
+    for (int64_t row = xcd_first_row(); row < a.n_rows; row += gridDim.x) {
         const PathGrid G = a.grid[a.row_path[row]];
         if (row - (int64_t)G.row_base >= G.h) continue;  // the empty-path row (k_path_out)
         const uint32_t gb = a.row_gbegin[row], ge = a.row_gend[row];
@@ -920,6 +940,7 @@ __global__ __launch_bounds__(WAVE) void k_tile_rows(TileArgs a) {
             }
             carry = fold;
             const float q = fminf(fabsf(mine + pa) * 256.0f, 255.0f);
+            if (a.tile_cov) a.tile_cov[(size_t)(row_tiles + nt) * 64 + lane] = fminf(fabsf(mine + pa), 1.0f);
             pa = 0.0f;
             ph = 0.0f;
             const uint32_t ti = row_tiles + nt;
@@ -1141,6 +1162,7 @@ struct RowTileArgs {
     const uint64_t* row_off;  // emit pass input
     int16_t* tile_xy;
     uint8_t* tile_alpha;
+    float* tile_cov;  // optional pre-quantisation coverage, 64 per tile
     int16_t* span_xy;
     uint16_t* span_w;
     Counters* cnt;
@@ -1151,7 +1173,6 @@ constexpr int RT_NT = RT_NW * WAVE;
 constexpr int RT_ROWS = TILE / RT_NW;  // pixel rows whose carry each wave takes
 static_assert(RT_ROWS * RT_NW == TILE && RT_ROWS <= 2, "the carry takes one or two pixel rows per wave");
 constexpr int RT_U = 2;  // entries per thread whose loads are issued together
-constexpr int RT_ROW_CHUNK = 16;  // consecutive tile rows on one XCD
 
 // The window's per-pixel sums are 64-bit fixed point (units of 2^-39): LDS
 // integer atomics cost a fraction of float ones, and the sums of the
@@ -1287,18 +1308,8 @@ __global__ __launch_bounds__(RT_NT) void k_row_tiles(RowTileArgs a) {
         }
         __syncthreads();
     }
-    // Neighbouring tile rows gather the same lines and run records: runs of
-    // RT_ROW_CHUNK consecutive rows go to the workgroups of one XCD
-    // (workgroups are dealt round-robin over the eight XCDs by id), so that
-    // the second gather hits that XCD's L2; the runs themselves go round-robin
-    // over the XCDs, which keeps their loads even.
     const int64_t grid_n = gridDim.x;
-    int64_t first_row = blockIdx.x;
-    if (grid_n % (8 * RT_ROW_CHUNK) == 0) {
-        const int64_t j = blockIdx.x / 8u;  // the workgroup's place among its XCD's
-        first_row = ((j / RT_ROW_CHUNK) * 8 + (int64_t)(blockIdx.x % 8u)) * RT_ROW_CHUNK + j % RT_ROW_CHUNK;
-    }
-    for (int64_t row = first_row; row < a.n_rows; row += grid_n) {
+    for (int64_t row = xcd_first_row(); row < a.n_rows; row += grid_n) {
         const PathGrid G = a.grid[a.row_path[row]];
         const int32_t r = (int32_t)(row - (int64_t)G.row_base);
         if (r >= G.h) continue;  // the empty-path row
@@ -1437,6 +1448,7 @@ __global__ __launch_bounds__(RT_NT) void k_row_tiles(RowTileArgs a) {
                     const int t = s.tlist[rk];
                     const int base = t * UWS + (int)quad * 4;
                     uint32_t word = 0;
+                    const uint64_t ti = (uint64_t)row_tiles + nt_w + rk;
                     const float cin = s.rtot[rk * 8 + (quad >> 1)];  // pixel row quad / 2
                     const float* fh = reinterpret_cast<const float*>(s.acc_h);
                     const float* fa = reinterpret_cast<const float*>(s.acc_a);
@@ -1445,10 +1457,10 @@ __global__ __launch_bounds__(RT_NT) void k_row_tiles(RowTileArgs a) {
                         const float av = fx ? fx_to_f(s.acc_a[base + k]) : fa[2 * (base + k)];
                         const float v = (cin + fh[2 * (base + k)]) + av;
                         word |= (uint32_t)fminf(fabsf(v) * 256.0f, 255.0f) << (8 * k);
+                        if (a.tile_cov) a.tile_cov[ti * 64 + quad * 4 + (uint32_t)k] = fminf(fabsf(v), 1.0f);
                         s.acc_a[base + k] = 0;  // cleared for the next window
                         s.acc_h[base + k] = 0;
                     }
-                    const uint64_t ti = (uint64_t)row_tiles + nt_w + rk;
                     reinterpret_cast<uint32_t*>(a.tile_alpha + ti * 64)[quad] = word;
                     if (quad == 0) {
                         a.tile_xy[2 * ti] = (int16_t)((G.tx0 + x0 + t) * TILE);
@@ -1469,7 +1481,7 @@ __global__ __launch_bounds__(RT_NT) void k_row_tiles(RowTileArgs a) {
 // Per-path output ranges and the empty-path tile.
 __global__ void k_path_out(const PathGrid* grid, int64_t n_paths, const uint64_t* row_off, const uint64_t* total,
                            int64_t* tstart, int32_t* tcount, int64_t* sstart, int32_t* scount, int16_t* tile_xy,
-                           uint8_t* tile_alpha, int64_t tile_base, int64_t span_base) {
+                           uint8_t* tile_alpha, float* tile_cov, int64_t tile_base, int64_t span_base) {
     int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (p >= n_paths) return;
     PathGrid g = grid[p];
@@ -1486,6 +1498,8 @@ __global__ void k_path_out(const PathGrid* grid, int64_t n_paths, const uint64_t
         tile_xy[2 * ti + 1] = 0;
         uint4* dst = reinterpret_cast<uint4*>(tile_alpha + ti * 64);
         for (int k = 0; k < 4; k++) dst[k] = make_uint4(0, 0, 0, 0);
+        if (tile_cov)
+            for (int k = 0; k < 64; k++) tile_cov[ti * 64 + k] = 0.0f;
     }
 }
 
@@ -1725,12 +1739,12 @@ static int run_plain_pass(Context* ctx, const SegInput& in, OutputDesc& out, hip
         ra.row_off = ctx->row_off.as<uint64_t>();
         ra.tile_xy = out.tile_xy;
         ra.tile_alpha = out.tile_alpha;
+        ra.tile_cov = out.tile_cov;
         ra.span_xy = out.span_xy;
         ra.span_w = out.span_w;
         ra.cnt = cnt;
-        int64_t row_blocks = std::min<int64_t>(ROWS, 1 << 16);
-        if (row_blocks >= 8 * RT_ROW_CHUNK) row_blocks -= row_blocks % (8 * RT_ROW_CHUNK);  // rows dealt by XCD
-        hipLaunchKernelGGL(k_row_tiles<false>, dim3((uint32_t)row_blocks), dim3(RT_NT), 0, s, ra);
+        const uint32_t row_blocks = row_grid(ROWS, 1 << 16);
+        hipLaunchKernelGGL(k_row_tiles<false>, dim3(row_blocks), dim3(RT_NT), 0, s, ra);
         HIPCHK(launch_scan<uint64_t>(ctx, ROWS, LoadU64{ra.row_cnt}, StoreU64{ctx->row_off.as<uint64_t>()}, OpAdd(),
                                      0ull, &tot[4], s));
         HIPCHK(hipGetLastError());
@@ -1763,11 +1777,11 @@ static int run_plain_pass(Context* ctx, const SegInput& in, OutputDesc& out, hip
             if (!ctx->ev[1]) HIPCHK(hipEventCreate(&ctx->ev[1]));
             HIPCHK(hipEventRecord(ctx->ev[0], s));
         }
-        hipLaunchKernelGGL(k_row_tiles<true>, dim3((uint32_t)row_blocks), dim3(RT_NT), 0, s, ra);
+        hipLaunchKernelGGL(k_row_tiles<true>, dim3(row_blocks), dim3(RT_NT), 0, s, ra);
         if (ctx->timing) HIPCHK(hipEventRecord(ctx->ev[1], s));
         hipLaunchKernelGGL(k_path_out, dim3(grid_for(NP, 256)), dim3(256), 0, s, grid, NP, ra.row_off,
                            (const uint64_t*)&tot[4], out.path_tile_start, out.path_tile_count, out.path_span_start,
-                           out.path_span_count, out.tile_xy, out.tile_alpha, out.tile_base, out.span_base);
+                           out.path_span_count, out.tile_xy, out.tile_alpha, out.tile_cov, out.tile_base, out.span_base);
         hipLaunchKernelGGL(k_fold_error, dim3(1), dim3(64), 0, s, cnt, ctx->call_error.as<uint32_t>());
         HIPCHK(hipGetLastError());
         if (ctx->timing) {
@@ -1854,6 +1868,7 @@ static int run_plain_pass(Context* ctx, const SegInput& in, OutputDesc& out, hip
     ta.row_off = ctx->row_off.as<uint64_t>();
     ta.tile_xy = out.tile_xy;
     ta.tile_alpha = out.tile_alpha;
+    ta.tile_cov = out.tile_cov;
     ta.span_xy = out.span_xy;
     ta.span_w = out.span_w;
     ta.cnt = cnt;
@@ -1878,11 +1893,11 @@ static int run_plain_pass(Context* ctx, const SegInput& in, OutputDesc& out, hip
         if (!ctx->ev[1]) HIPCHK(hipEventCreate(&ctx->ev[1]));
         HIPCHK(hipEventRecord(ctx->ev[0], s));
     }
-    hipLaunchKernelGGL(k_tile_rows, dim3((uint32_t)std::min<int64_t>(ROWS, 1 << 20)), dim3(WAVE), 0, s, ta);
+    hipLaunchKernelGGL(k_tile_rows, dim3(row_grid(ROWS, 1 << 20)), dim3(WAVE), 0, s, ta);
     if (ctx->timing) HIPCHK(hipEventRecord(ctx->ev[1], s));
     hipLaunchKernelGGL(k_path_out, dim3(grid_for(NP, 256)), dim3(256), 0, s, grid, NP, ta.row_off,
                        (const uint64_t*)&tot[4], out.path_tile_start, out.path_tile_count, out.path_span_start,
-                       out.path_span_count, out.tile_xy, out.tile_alpha, out.tile_base, out.span_base);
+                       out.path_span_count, out.tile_xy, out.tile_alpha, out.tile_cov, out.tile_base, out.span_base);
     hipLaunchKernelGGL(k_fold_error, dim3(1), dim3(64), 0, s, cnt, ctx->call_error.as<uint32_t>());
     HIPCHK(hipGetLastError());
     if (ctx->timing) {
@@ -1981,6 +1996,7 @@ int run_plain(Context* ctx, const SegInput& in, OutputDesc& out, hipStream_t s, 
         OutputDesc od;
         od.tile_xy = out.tile_xy + (room ? 2 * T : 0);
         od.tile_alpha = out.tile_alpha + (room ? 64 * T : 0);
+        od.tile_cov = out.tile_cov ? out.tile_cov + (room ? 64 * T : 0) : nullptr;
         od.tile_capacity = room ? out.tile_capacity - T : 0;
         od.span_xy = out.span_xy + (room ? 2 * S : 0);
         od.span_w = out.span_w + (room ? S : 0);
