@@ -246,4 +246,21 @@ __device__ inline bool dda_step(Dda& d) {
     return c == 1.0f;
 }
 
+// k steps of dda_step in select form (both candidate updates computed, one
+// kept: the same f32 operations on the same values, so the same state), for
+// replaying a walk up to one of its increments: the caller keeps k below the
+// line's increment count, so no replayed step is the terminating one.
+__device__ inline void dda_skip(Dda& d, uint32_t k) {
+    for (uint32_t i = 0; i < k; i++) {
+        const bool take_y = d.row_t1 < d.col_t1;
+        const float c = take_y ? d.row_t1 : d.col_t1;
+        const float r = fminf(d.row_t1 + d.y_step, 1.0f), q = fminf(d.col_t1 + d.x_step, 1.0f);
+        d.row_t1 = take_y ? r : d.row_t1;
+        d.col_t1 = take_y ? d.col_t1 : q;
+        d.y = take_y ? wrap16(d.y + d.y_dir) : d.y;
+        d.x = take_y ? d.x : wrap16(d.x + d.x_dir);
+        d.t0 = c;
+    }
+}
+
 }  // namespace ochre
