@@ -654,7 +654,6 @@ namespace {
 int verb_npts(uint8_t v) { return ochre::verb_points(v); }
 
 struct DevPath {
-    DBuf s_verbs, s_points, s_weights, s_voff, s_poff, s_woff, s_width, s_xf;
     DBuf tile_xy, tile_alpha, span_xy, span_w, pts, ptc, pss, psc;
     // the path's arrays, packed into one pinned host buffer and sent with one copy
     void* stage_host = nullptr;
@@ -704,16 +703,6 @@ static int upload_packed(DevPath* dp, const HostPiece* pc, int n, void** dev, hi
     return e == hipSuccess ? 0 : map_hip_error(e);
 }
 
-template <typename T>
-int upload(DBuf& b, const std::vector<T>& v, hipStream_t s) {
-    hipError_t e = b.ensure(std::max<size_t>(v.size() * sizeof(T), 16));
-    if (e != hipSuccess) return map_hip_error(e);
-    if (!v.empty()) {
-        e = hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s);
-        if (e != hipSuccess) return map_hip_error(e);
-    }
-    return 0;
-}
 
 // Runs the recorded strokes through the device expansion (flatten + stroke)
 // and splices each outline into its placeholder segment.  The outlines come
@@ -722,27 +711,33 @@ int upload(DBuf& b, const std::vector<T>& v, hipStream_t s) {
 int expand_strokes(ochre_context* ctx, ochre_rasterizer* r, DevPath* dp, hipStream_t s) {
     const int64_t K = (int64_t)r->s_width.size();
     std::vector<Xform> xf((size_t)K, Xform{1, 0, 0, 1, 0, 0});
-    int st;
-    if ((st = upload(dp->s_verbs, r->s_verbs, s)) || (st = upload(dp->s_points, r->s_points, s)) ||
-        (st = upload(dp->s_weights, r->s_weights, s)) || (st = upload(dp->s_voff, r->s_voff, s)) ||
-        (st = upload(dp->s_poff, r->s_poff, s)) || (st = upload(dp->s_woff, r->s_woff, s)) ||
-        (st = upload(dp->s_width, r->s_width, s)) || (st = upload(dp->s_xf, xf, s)))
-        return st;
+    // the recorded strokes in one packed upload
+    const HostPiece pc[8] = {{r->s_verbs.data(), r->s_verbs.size()},
+                             {r->s_points.data(), r->s_points.size() * sizeof(float)},
+                             {r->s_weights.data(), r->s_weights.size() * sizeof(float)},
+                             {r->s_voff.data(), r->s_voff.size() * sizeof(int64_t)},
+                             {r->s_poff.data(), r->s_poff.size() * sizeof(int64_t)},
+                             {r->s_woff.data(), r->s_woff.size() * sizeof(int64_t)},
+                             {r->s_width.data(), r->s_width.size() * sizeof(float)},
+                             {xf.data(), xf.size() * sizeof(Xform)}};
+    void* dv[8];
+    int st = upload_packed(dp, pc, 8, dv, s);
+    if (st) return st;
     SegInput si;
     si.n_segs = K;
     si.n_paths = K;
-    si.verbs = dp->s_verbs.as<uint8_t>();
-    si.points = dp->s_points.as<float>();
-    si.weights = dp->s_weights.as<float>();
-    si.seg_verb_off = dp->s_voff.as<int64_t>();
-    si.seg_point_off = dp->s_poff.as<int64_t>();
-    si.seg_weight_off = dp->s_woff.as<int64_t>();
-    si.seg_xf = dp->s_xf.as<Xform>();
+    si.verbs = static_cast<const uint8_t*>(dv[0]);
+    si.points = static_cast<const float*>(dv[1]);
+    si.weights = static_cast<const float*>(dv[2]);
+    si.seg_verb_off = static_cast<const int64_t*>(dv[3]);
+    si.seg_point_off = static_cast<const int64_t*>(dv[4]);
+    si.seg_weight_off = static_cast<const int64_t*>(dv[5]);
+    si.seg_xf = static_cast<const Xform*>(dv[7]);
     si.n_verbs = (int64_t)r->s_verbs.size();
     si.n_points = (int64_t)r->s_points.size() / 2;
     si.n_weights = (int64_t)r->s_weights.size();
     StrokeOutline o;
-    if ((st = stroke_expand(&ctx->c, si, dp->s_width.as<float>(), nullptr, o, s))) return st;
+    if ((st = stroke_expand(&ctx->c, si, static_cast<const float*>(dv[6]), nullptr, o, s))) return st;
     std::vector<int64_t> ov((size_t)K + 1), op((size_t)K + 1);
     std::vector<uint8_t> overbs((size_t)o.in.n_verbs);
     std::vector<float> opts((size_t)o.in.n_points * 2);
@@ -1011,7 +1006,8 @@ int ochre_rasterizer_finish(ochre_rasterizer* r, const ochre_tile_builder* build
         si.n_points = (int64_t)r->points.size() / 2;
         si.n_weights = (int64_t)r->weights.size();
         OutputDesc od;
-        const bool one_fill = nseg == 1 && !r->seg_raw[0] && r->s_width.empty();
+        // one fill, or one stroke (its outline, expanded above, is filled)
+        const bool one_fill = nseg == 1 && !r->seg_raw[0];
         if (one_fill && !ctx->c.timing) {
             SegInput s1 = si;
             s1.seg_path = nullptr;

@@ -1,5 +1,5 @@
 """The drop-in Rasterizer's finish paths against the oracle Rasterizer, event
-for event: a single fill (one upload, one launch, one read-back), and
+for event: a single fill or a single stroke (one upload, one launch), and
 finish_many over a sequence mixing single fills, single strokes, empty
 rasterizers and one with several fills in the middle (the demo's loop,
 examples/svg.rs:140-156, finished into one builder)."""
@@ -69,6 +69,20 @@ def test_single_fills_one_by_one(gpu_ctx):
         r.finish(rec)
         events += rec.events
     _same(events, _oracle_events(kinds, b), "single fills")
+
+
+def test_single_strokes_one_by_one(gpu_ctx):
+    """A Rasterizer holding one stroke: its outline, expanded on the device, is
+    finished as a single fill is."""
+    from ochre_amd import EventRecorder
+    b = W.with_widths(W.concat([W.glyph_run(30, seed=56), W.random_paths(20, 57)]), 0.5, 3.0, 10)
+    kinds = [("stroke", i) for i in range(b.n_paths)]
+    events = []
+    for r in _make(kinds, b):
+        rec = EventRecorder()
+        r.finish(rec)
+        events += rec.events
+    _same(events, _oracle_events(kinds, b), "single strokes")
 
 
 def test_finish_many_matches_one_by_one_and_oracle(gpu_ctx):
