@@ -163,7 +163,9 @@ def api_latency(reps=20):
     """The drop-in Rasterizer API on the README path: (1) fill + finish with
     the TileBuilder callbacks in Python (what a Python caller pays), and (2)
     ochre_rasterizer_finish alone with a NULL builder, timed around the C
-    call (upload, device pipeline, download: the library's own cost)."""
+    call (upload, device pipeline, download: the library's own cost), on the
+    README path and on three glyphs of the glyph run."""
+    import numpy as np
     from ochre_amd import EventRecorder, Rasterizer, _lib
     from ochre_amd import workloads as W
     b = W.readme_path()
@@ -178,19 +180,29 @@ def api_latency(reps=20):
         times.append(time.perf_counter() - t)
     times = sorted(times[3:])
     lib = _lib.load()
-    vv = v.ctypes.data_as(ctypes.c_void_p)
-    pp = p.ctypes.data_as(ctypes.c_void_p)
-    t = _lib.Transform((ctypes.c_float * 4)(*xf[:4]), (ctypes.c_float * 2)(*xf[4:]))
-    fin = []
-    for _ in range(reps + 3):
-        h = lib.ochre_rasterizer_new()
-        _lib.check(lib.ochre_rasterizer_fill(h, vv, v.size, pp, p.shape[0], None, 0, ctypes.byref(t)), "fill")
-        a = time.perf_counter()
-        _lib.check(lib.ochre_rasterizer_finish(h, None), "finish")
-        fin.append(time.perf_counter() - a)
-    fin = sorted(fin[3:])
-    return {"ms": times[len(times) // 2] * 1e3, "finish_c_us": fin[len(fin) // 2] * 1e6,
-            "workload": "readme_path (1 path, 100 tiles, 23 spans)",
+
+    def finish_c_us(v, p, xf):
+        v = np.ascontiguousarray(v)
+        p = np.ascontiguousarray(p)
+        vv = v.ctypes.data_as(ctypes.c_void_p)
+        pp = p.ctypes.data_as(ctypes.c_void_p)
+        t = _lib.Transform((ctypes.c_float * 4)(*xf[:4]), (ctypes.c_float * 2)(*xf[4:]))
+        fin = []
+        for _ in range(reps + 3):
+            h = lib.ochre_rasterizer_new()
+            _lib.check(lib.ochre_rasterizer_fill(h, vv, v.size, pp, p.shape[0], None, 0, ctypes.byref(t)), "fill")
+            a = time.perf_counter()
+            _lib.check(lib.ochre_rasterizer_finish(h, None), "finish")
+            fin.append(time.perf_counter() - a)
+        fin = sorted(fin[3:])
+        return fin[len(fin) // 2] * 1e6
+
+    g = W.glyph_run(64, seed=5)
+    glyphs = {f"glyph_run(64, seed=5)[{i}] ({int(g.verb_off[i + 1] - g.verb_off[i])} verbs)":
+              finish_c_us(*g.path(i)[:2], g.path(i)[3]) for i in range(3)}
+    return {"ms": times[len(times) // 2] * 1e3, "finish_c_us": finish_c_us(v, p, xf),
+            "finish_c_us_glyphs": glyphs,
+            "workload": "readme_path (1 path, 100 tiles, 23 spans); single glyphs",
             "what": "ms: Rasterizer.fill + finish with Python TileBuilder callbacks; finish_c_us: "
                     "ochre_rasterizer_finish with a NULL builder (C ABI, no callbacks); medians"}
 
