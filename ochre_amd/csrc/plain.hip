@@ -1269,11 +1269,14 @@ __device__ inline void rt_load_lines(RtBatch& b, const RowTileArgs& a) {
         b.L[u] = !(b.vv[u] >> 31) ? a.lines[b.rr[u].line & 0x7fffffffu] : make_float4(0, 0, 0, 0);
 }
 
-// One batch of a window's entries: occupancy and crossing sums from the keys,
-// and (emit pass) the runs replayed.
+// One batch of a window's entries: occupancy (into the thread's bits, reduced
+// per wave after the window's pass: every run of a wave would otherwise hit the
+// one occupancy word) and crossing sums from the keys, and (emit pass) the
+// runs replayed.
+static_assert(UW == 32, "one occupancy word per window");
 template <bool EMIT>
 __device__ inline void rt_entries(RowLds<EMIT>& s, const RtBatch& b, uint32_t e0, uint32_t ee, uint32_t xmask,
-                                  int32_t x0, bool fx, Counters* cnt) {
+                                  int32_t x0, bool fx, Counters* cnt, uint32_t& occ_bits) {
 #pragma unroll
     for (int u = 0; u < RT_U; u++) {
         if (e0 + (uint32_t)(u * RT_NT) >= ee) break;
@@ -1282,7 +1285,7 @@ __device__ inline void rt_entries(RowLds<EMIT>& s, const RtBatch& b, uint32_t e0
             const int sg = (int)(int8_t)(b.vv[u] & 0xffu);
             if (sg) atomicAdd(&s.tsign[x], sg);
         } else {
-            atomicOr(&s.occ[x >> 5], 1u << (x & 31));
+            occ_bits |= 1u << x;
             if constexpr (EMIT) rt_replay_run(s, x, b.rr[u], b.L[u], fx, cnt);
         }
     }
@@ -1352,9 +1355,9 @@ __global__ __launch_bounds__(RT_NT) void k_row_tiles(RowTileArgs a) {
             if (tid < UW / 32) s.occ[tid] = 0;
             __syncthreads();
             // one pass over the window's entries, a batch at a time
-            uint32_t e0 = eb + (uint32_t)tid;
+            uint32_t e0 = eb + (uint32_t)tid, occ_bits = 0;
             if (pf) {
-                rt_entries(s, nb, e0, ee, xmask, x0, fx, a.cnt);
+                rt_entries(s, nb, e0, ee, xmask, x0, fx, a.cnt, occ_bits);
                 e0 += RT_U * RT_NT;
             }
             for (; e0 < ee; e0 += RT_U * RT_NT) {
@@ -1364,8 +1367,10 @@ __global__ __launch_bounds__(RT_NT) void k_row_tiles(RowTileArgs a) {
                     rt_load_recs(b, a);
                     rt_load_lines(b, a);
                 }
-                rt_entries(s, b, e0, ee, xmask, x0, fx, a.cnt);
+                rt_entries(s, b, e0, ee, xmask, x0, fx, a.cnt, occ_bits);
             }
+            for (int o = 1; o < WAVE; o <<= 1) occ_bits |= __shfl_xor(occ_bits, o, WAVE);
+            if (lane == 0 && occ_bits) atomicOr(&s.occ[0], occ_bits);
             __syncthreads();
             if (has_next) rt_load_keys(nb, a, nbeg + (uint32_t)tid, nend);
             if (wid == 0) {
