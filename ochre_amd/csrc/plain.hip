@@ -236,9 +236,10 @@ __global__ __launch_bounds__(256) void k_flatten(FlattenArgs a) {
         });
         return;
     }
-    // count pass: lines per command, and the path's line bounding box.  A
-    // path's commands are contiguous, so the wave reduces the box per path
-    // (segmented by path) and one lane per path segment does the atomics.
+    // count pass: lines per command, and a box holding the path's lines (from
+    // the commands' control points: command_count).  A path's commands are
+    // contiguous, so the wave reduces the box per path (segmented by path) and
+    // one lane per path segment does the atomics.
     int32_t key = -1;
     uint32_t n = 0, flags = 0;
     BoxAcc box;
@@ -248,11 +249,7 @@ __global__ __launch_bounds__(256) void k_flatten(FlattenArgs a) {
         if (c.bad) {
             atomicOr(&a.cnt->error, (uint32_t)ERR_INPUT);
         } else {
-            const bool ok = command_lines(c, [&](V2 p0, V2 p1) {
-                n++;
-                box.add(p0);
-                box.add(p1);
-            });
+            const bool ok = command_count(c, n, [&](V2 q) { box.add(q); });
             if (!ok) atomicOr(&a.cnt->error, (uint32_t)ERR_FLATTEN);
         }
         a.cmd_lines[i] = n;

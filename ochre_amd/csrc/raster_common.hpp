@@ -88,6 +88,68 @@ __device__ inline bool command_lines(const Cmd& c, Emit&& emit) {
     return ok;
 }
 
+// command_lines' count of lines, visiting (add) points whose box holds every
+// line's end points: a quadratic's or cubic's control points instead of its
+// evaluated points (their convex hull holds the curve; the grid's margin takes
+// the evaluation's rounding), so a curve's lines are counted by the
+// reference's t loop alone; a conic is flattened as command_lines does.
+template <class Add>
+__device__ inline bool command_count(const Cmd& c, uint32_t& n, Add&& add) {
+    bool ok = true;
+    V2 last = c.start;
+    V2 first = c.first;
+    switch (c.verb) {
+    case MOVE:
+        if (!veq(last, first)) {
+            n++;
+            add(last);
+            add(first);
+        }
+        last = c.pt[0];
+        first = c.pt[0];
+        break;
+    case LINE:
+        n++;
+        add(last);
+        add(c.pt[0]);
+        last = c.pt[0];
+        break;
+    case QUAD:
+    case CUBIC: {
+        const bool cubic = c.verb == CUBIC;
+        const float dt = cubic ? cubic_dt(last, c.pt[0], c.pt[1], c.pt[2]) : quad_dt(last, c.pt[0], c.pt[1]);
+        bool limited = false;
+        n += t_steps(dt, &limited);
+        if (limited) ok = false;
+        add(last);
+        add(c.pt[0]);
+        add(c.pt[1]);
+        if (cubic) add(c.pt[2]);
+        last = cubic ? cubic_at(1.0f, last, c.pt[0], c.pt[1], c.pt[2]) : quad_at(1.0f, last, c.pt[0], c.pt[1]);
+        break;
+    }
+    case CONIC: {
+        V2 prev = last;
+        ok = conic_walk(last, c.pt[0], c.pt[1], c.w, [&](V2 q) {
+            n++;
+            add(prev);
+            add(q);
+            prev = q;
+        });
+        last = prev;
+        break;
+    }
+    default:
+        break;
+    }
+    if (c.last_in_path && !veq(last, first)) {
+        n++;
+        add(last);
+        add(first);
+    }
+    return ok;
+}
+
 constexpr float HUGE_COORD = 30000.0f;
 enum BoxFlags : uint32_t { BOX_FULL = 1u, BOX_LINES = 2u };
 
