@@ -220,20 +220,115 @@ struct FlattenArgs {
     Counters* cnt;
 };
 
+// A command of the emit pass's block, for the threads that make its lines:
+// its points, a curve's step, and command_lines' decomposition of its lines
+// (a Move's implicit close or a Line's own line, a curve's segments, the
+// path's final close).
+struct FlatCmd {
+    float2 start, p0, p1, p2, first;
+    float dt;
+    uint32_t info;  // verb | segments << 4 (up to FLATTEN_LIMIT) | own << 30 | close << 31
+    uint32_t path;
+};
+
 template <bool EMIT>
 __global__ __launch_bounds__(256) void k_flatten(FlattenArgs a) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     const bool in_range = i < a.in.n_verbs;
     if (EMIT) {
-        if (!in_range) return;
-        Cmd c = resolve_cmd(a.in, a.scan, i);
-        if (c.bad) return;
-        uint32_t o = a.cmd_line_off[i];
-        command_lines(c, [&](V2 p0, V2 p1) {
-            a.lines[o] = make_float4(p0.x, p0.y, p1.x, p1.y);
-            a.line_path[o] = (uint32_t)c.path;
-            o++;
-        });
+        // A thread per command resolves it (a conic's lines it writes itself);
+        // then a thread per line of the block's commands makes its line, so
+        // that consecutive threads write consecutive lines.
+        __shared__ FlatCmd fc[256];
+        __shared__ uint32_t foff[257];
+        const int t = threadIdx.x;
+        const int64_t i0 = (int64_t)blockIdx.x * blockDim.x;
+        const int nb = (int)min<int64_t>((int64_t)blockDim.x, a.in.n_verbs - i0);
+        if (in_range) {
+            const Cmd c = resolve_cmd(a.in, a.scan, i);
+            const uint32_t o = a.cmd_line_off[i];
+            foff[t] = o;
+            if (t == nb - 1) foff[nb] = o + a.cmd_lines[i];
+            FlatCmd f;
+            f.start = make_float2(c.start.x, c.start.y);
+            f.p0 = make_float2(c.pt[0].x, c.pt[0].y);
+            f.p1 = make_float2(c.pt[1].x, c.pt[1].y);
+            f.p2 = make_float2(c.pt[2].x, c.pt[2].y);
+            f.first = make_float2(c.first.x, c.first.y);
+            f.dt = 0.0f;
+            f.path = (uint32_t)c.path;
+            uint32_t own = 0, nseg = 0;
+            V2 last = c.start, first = c.first;
+            if (c.bad) {
+                f.info = 0xfu;  // no lines (the count pass reported it)
+            } else if (c.verb == CONIC) {
+                uint32_t q = o;
+                command_lines(c, [&](V2 p0, V2 p1) {
+                    a.lines[q] = make_float4(p0.x, p0.y, p1.x, p1.y);
+                    a.line_path[q] = (uint32_t)c.path;
+                    q++;
+                });
+                f.info = (uint32_t)CONIC;
+            } else {
+                if (c.verb == MOVE) {
+                    own = veq(last, first) ? 0u : 1u;
+                    last = first = c.pt[0];
+                } else if (c.verb == LINE) {
+                    own = 1u;
+                    last = c.pt[0];
+                } else if (c.verb == QUAD || c.verb == CUBIC) {
+                    const bool cubic = c.verb == CUBIC;
+                    f.dt = cubic ? cubic_dt(last, c.pt[0], c.pt[1], c.pt[2]) : quad_dt(last, c.pt[0], c.pt[1]);
+                    bool limited = false;
+                    nseg = t_steps(f.dt, &limited);
+                    last = cubic ? cubic_at(1.0f, last, c.pt[0], c.pt[1], c.pt[2]) : quad_at(1.0f, last, c.pt[0], c.pt[1]);
+                }
+                const uint32_t close = (c.last_in_path && !veq(last, first)) ? 1u : 0u;
+                f.info = (uint32_t)c.verb | (nseg << 4) | (own << 30) | (close << 31);
+            }
+            fc[t] = f;
+        }
+        __syncthreads();
+        if (nb <= 0) return;
+        const uint32_t l0 = foff[0], l1 = foff[nb];
+        for (uint32_t q = l0 + (uint32_t)t; q < l1; q += blockDim.x) {
+            int lo = 0, hi = nb - 1;  // the last command with foff <= q (it has lines)
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (foff[mid] <= q) lo = mid;
+                else hi = mid - 1;
+            }
+            const FlatCmd f = fc[lo];
+            const int verb = (int)(f.info & 0xfu);
+            if (verb == CONIC || verb == 0xf) continue;  // written above / none
+            const uint32_t j = q - foff[lo], nseg = (f.info >> 4) & 0x3ffffffu, own = (f.info >> 30) & 1u;
+            const V2 c0 = mk(f.start.x, f.start.y), fp = mk(f.first.x, f.first.y);
+            const V2 c1 = mk(f.p0.x, f.p0.y), c2 = mk(f.p1.x, f.p1.y), c3 = mk(f.p2.x, f.p2.y);
+            V2 pa, pb;
+            if (verb == QUAD || verb == CUBIC) {
+                const bool cubic = verb == CUBIC;
+                if (j < nseg) {  // segment j: t accumulated the reference's way (path.rs:49-74)
+                    float tt = 0.0f, tp = 0.0f;
+                    for (uint32_t k = 0; k <= j; k++) {
+                        tp = tt;
+                        tt = fminf(tt + f.dt, 1.0f);
+                    }
+                    pa = j == 0 ? c0 : (cubic ? cubic_at(tp, c0, c1, c2, c3) : quad_at(tp, c0, c1, c2));
+                    pb = cubic ? cubic_at(tt, c0, c1, c2, c3) : quad_at(tt, c0, c1, c2);
+                } else {  // the path's final close from the curve's end
+                    pa = cubic ? cubic_at(1.0f, c0, c1, c2, c3) : quad_at(1.0f, c0, c1, c2);
+                    pb = fp;
+                }
+            } else if (own && j == 0) {  // a Move's implicit close, or a Line
+                pa = c0;
+                pb = verb == MOVE ? fp : c1;
+            } else {  // the path's final close, from a Line's point or the command's start (a Close)
+                pa = verb == LINE ? c1 : c0;
+                pb = fp;
+            }
+            a.lines[q] = make_float4(pa.x, pa.y, pb.x, pb.y);
+            a.line_path[q] = f.path;
+        }
         return;
     }
     // count pass: lines per command, and a box holding the path's lines (from
