@@ -357,8 +357,13 @@ int ochre_coverage_adjoint(ochre_context* ctx, const ochre_path_batch* in, const
     t.span_count = target->span_count;
     t.span_xy = target->span_xy;
     t.span_w = target->span_width;
-    return run_adjoint(&ctx->c, si, t, loss, d_points, d_transforms, reinterpret_cast<hipStream_t>(stream),
-                       ctx->c.batched_waves_per_cu, ctx->c.batched_gran, ctx->c.batched_tc);
+    // The adjoint's own launch shape, not the batched call's tuned one (which
+    // is tuned for the ordered kernel behind the dense ones and may leave it a
+    // wave per CU): on 4,096 16-cubic glyphs 32-tile chunks, a path per pop and
+    // 8 or more waves per CU measured 1.58-1.61 ms, 16-tile chunks 2.37-2.42,
+    // one wave per CU 6.6-9.8 ms.
+    return run_adjoint(&ctx->c, si, t, loss, d_points, d_transforms, reinterpret_cast<hipStream_t>(stream), 8, 1,
+                       32);
 }
 
 int ochre_autotune_batched(ochre_context* ctx, const ochre_path_batch* in, ochre_batch_output* out, void* stream,
