@@ -585,7 +585,7 @@ __global__ __launch_bounds__(256) void k_dda(DdaArgs a) {
         s.recs = a.recs;
         s.cnt = a.cnt;
         s.unordered = a.unordered;
-        walk_line(mk(L.x, L.y), mk(L.z, L.w), s);
+        if (!walk_line_tiles(mk(L.x, L.y), mk(L.z, L.w), s)) walk_line(mk(L.x, L.y), mk(L.z, L.w), s);
     }
 }
 

@@ -299,6 +299,77 @@ __device__ inline bool line_tiles(V2 last, V2 point, uint32_t& incs, Tile&& tile
     return true;
 }
 
+// walk_line's runs and tile-row crossings (for sinks that take no increments)
+// at tile granularity, for the lines chain_walk_ok covers: line_tiles' chains,
+// with the other axis' chain advanced to each tile event, which gives the
+// state one step before the run the event starts (the x step is taken on a tie:
+// the y steps before an x event are those with a smaller parameter, the x steps
+// before a y event those with a smaller or equal one).  Emits the runs and
+// crossings in walk_line's order.  Returns false, emitting nothing, for the
+// lines it does not cover.
+template <class Sink>
+__device__ inline bool walk_line_tiles(V2 last, V2 point, Sink& sink) {
+    if (!chain_walk_ok(last, point)) return false;
+    if (veq(point, last)) return true;  // rasterizer.rs:73
+    Dda d;
+    dda_init(d, last, point);
+    const int32_t x0 = d.x, y0 = d.y, xd = d.x_dir, yd = d.y_dir;
+    int32_t tx = x0 >> 3, ty = y0 >> 3;
+    sink.run((int16_t)tx, (int16_t)ty, true, d);
+    auto first_k = [](int32_t p, int32_t dir) { return (uint32_t)(dir > 0 ? 8 - (p & 7) : (p & 7) + 1); };
+    auto value_at = [](float c, uint32_t i, uint32_t target, float step) {
+        for (; i < target; i++) c = fminf(c + step, 1.0f);
+        return c;
+    };
+    uint32_t ix = 0, iy = 0;  // steps taken along each axis, and the next step's parameter
+    float cx = d.col_t1, cy = d.row_t1;
+    uint32_t exi = xd ? first_k(x0, xd) - 1u : 0u, eyi = yd ? first_k(y0, yd) - 1u : 0u;  // the events' step indices
+    Dda b = d;
+    for (;;) {
+        const float exv = xd ? value_at(cx, ix, exi, d.x_step) : INFINITY;
+        const float eyv = yd ? value_at(cy, iy, eyi, d.y_step) : INFINITY;
+        const bool xe = exv < 1.0f, ye = eyv < 1.0f;
+        if (xe && (!ye || exv <= eyv)) {
+            while (cy < exv) {
+                cy = fminf(cy + d.y_step, 1.0f);
+                iy++;
+            }
+            b.x = (int16_t)(x0 + xd * (int32_t)exi);
+            b.y = (int16_t)(y0 + yd * (int32_t)iy);
+            b.row_t1 = cy;
+            b.col_t1 = exv;
+            tx += xd;
+            sink.run((int16_t)tx, (int16_t)ty, false, b);
+            ix = exi + 1u;
+            cx = fminf(exv + d.x_step, 1.0f);
+            exi += 8u;
+        } else if (ye) {
+            while (cx <= eyv) {
+                cx = fminf(cx + d.x_step, 1.0f);
+                ix++;
+            }
+            b.x = (int16_t)(x0 + xd * (int32_t)ix);
+            b.y = (int16_t)(y0 + yd * (int32_t)eyi);
+            b.row_t1 = eyv;
+            b.col_t1 = cx;
+            const int32_t nty = ty + yd;
+            sink.crossing((int16_t)tx, (int16_t)(nty < ty ? nty : ty), (int8_t)yd);
+            ty = nty;
+            sink.run((int16_t)tx, (int16_t)ty, false, b);
+            iy = eyi + 1u;
+            cy = fminf(eyv + d.y_step, 1.0f);
+            eyi += 8u;
+        } else {
+            break;
+        }
+    }
+    const int32_t ty_end = (int32_t)f2i16(floorf(point.y)) >> 3;  // the snap (rasterizer.rs:118-131)
+    if (ty_end != ty)
+        sink.crossing((int16_t)((int32_t)f2i16(floorf(point.x)) >> 3), (int16_t)(ty_end < ty ? ty_end : ty),
+                      (int8_t)(int16_t)(ty_end - ty));
+    return true;
+}
+
 // Tile grid of a path from the bounding box (pixel floors) of all its line
 // end points, widened by the DDA's worst-case overshoot: the crossing
 // parameters are summed sequentially (rasterizer.rs:110,114), so after k
