@@ -390,74 +390,60 @@ __device__ void fused_path(const FusedIn& in, const FusedOut& out, FusedCtl* ctl
     __syncthreads();
     bool bbox_err = false, walk_err = false;
     {
-        uint32_t next = 0;
-        bool active = false;
-        Dda d;
-        uint32_t line = 0, cnt = 0, steps = 0;
-        int16_t ty_prev = 0, cur_tx = 0, cur_ty = 0;
-        V2 point = mk(0.0f, 0.0f);
-        LineWindow win;
-        win.load(lines, nullptr, L, 0, lane);
-        while (next < L || __ballot(active)) {
-            const uint64_t need = __ballot(!active);
-            const uint32_t my = next + (uint32_t)__popcll(need & lt);
-            float4 ln = make_float4(0, 0, 0, 0);
-            if (need && next < L) ln = win.get(my, nullptr, false);
-            if (!active && my < L) {
-                line = my;
-                V2 a = mk(ln.x, ln.y);
-                point = mk(ln.z, ln.w);
-                if (veq(point, a)) {
-                    incoff[line] = 0;
-                } else {
-                    dda_init(d, a, point);
-                    ty_prev = div8(d.y);
-                    cur_tx = div8(d.x);
-                    cur_ty = ty_prev;
-                    int lx = cur_tx - tx0, ly = cur_ty - ty0;
-                    if ((unsigned)lx < (unsigned)GW && (unsigned)ly < (unsigned)GH) atomicOr(&s.occ[ly], 1u << lx);
-                    else bbox_err = true;
-                    cnt = 0;
-                    steps = 0;
-                    active = true;
-                }
+        // a lane per line at tile granularity (line_tiles: the tiles, tile-row
+        // crossings and increments from the DDA's two crossing-parameter
+        // chains); a line it does not cover is walked pixel by pixel
+        auto mark = [&](int32_t tx, int32_t ty) {
+            const int lx = tx - tx0, ly = ty - ty0;
+            if ((unsigned)lx < (unsigned)GW && (unsigned)ly < (unsigned)GH) atomicOr(&s.occ[ly], 1u << lx);
+            else bbox_err = true;
+        };
+        auto cross = [&](int32_t tx, int32_t ty, int sign) {
+            const int lx = tx - tx0, ly = ty - ty0;
+            if ((unsigned)lx < (unsigned)GW && (unsigned)ly < (unsigned)GH) atomicAdd(&s.u.tsign[ly * F_GRID + lx], sign);
+            else bbox_err = true;
+        };
+        for (uint32_t line = (uint32_t)lane; line < L; line += WAVE) {
+            const float4 ln = lines[line];
+            const V2 a = mk(ln.x, ln.y), point = mk(ln.z, ln.w);
+            uint32_t cnt = 0;
+            if (line_tiles(a, point, cnt, mark, cross)) {
+                incoff[line] = cnt;
+                continue;
             }
-            next += (uint32_t)__popcll(need);
-            win.advance(lines, nullptr, L, next, lane);
-            if (active) {
+            if (veq(point, a)) {  // rasterizer.rs:73
+                incoff[line] = 0;
+                continue;
+            }
+            Dda d;
+            dda_init(d, a, point);
+            int16_t ty_prev = div8(d.y), cur_tx = div8(d.x), cur_ty = ty_prev;
+            mark(cur_tx, cur_ty);
+            for (uint32_t steps = 0;;) {
                 cnt++;
                 bool term = dda_step(d);
-                if (term) {
+                if (term) {  // rasterizer.rs:118-121
                     d.x = f2i16(floorf(point.x));
                     d.y = f2i16(floorf(point.y));
                 }
-                int16_t ty = div8(d.y);
-                if (ty != ty_prev) {
-                    int lx = div8(d.x) - tx0, ly = (ty_prev < ty ? ty_prev : ty) - ty0;
-                    if ((unsigned)lx < (unsigned)GW && (unsigned)ly < (unsigned)GH)
-                        atomicAdd(&s.u.tsign[ly * F_GRID + lx], (int)(int8_t)(int16_t)(ty - ty_prev));
-                    else bbox_err = true;
+                const int16_t ty = div8(d.y);
+                if (ty != ty_prev) {  // rasterizer.rs:123-131
+                    cross(div8(d.x), ty_prev < ty ? ty_prev : ty, (int)(int8_t)(int16_t)(ty - ty_prev));
                     ty_prev = ty;
                 }
                 if (!term && ++steps >= WALK_LIMIT) {
                     walk_err = true;
                     term = true;
                 }
-                if (term) {
-                    incoff[line] = cnt;
-                    active = false;
-                } else {
-                    int16_t tx = div8(d.x);
-                    if (tx != cur_tx || ty != cur_ty) {
-                        cur_tx = tx;
-                        cur_ty = ty;
-                        int lx = tx - tx0, ly = ty - ty0;
-                        if ((unsigned)lx < (unsigned)GW && (unsigned)ly < (unsigned)GH)
-                            atomicOr(&s.occ[ly], 1u << lx);
-                        else bbox_err = true;
-                    }
+                if (term) break;
+                const int16_t tx = div8(d.x);
+                if (tx != cur_tx || ty != cur_ty) {
+                    cur_tx = tx;
+                    cur_ty = ty;
+                    mark(tx, ty);
                 }
             }
+            incoff[line] = cnt;
         }
     }
     __syncthreads();
