@@ -1286,8 +1286,36 @@ struct RowLds {
     uint32_t nt, ns, nocc;
 };
 
+// The increments of a run from the DDA state at its first one (p0: that
+// increment's start point), into the window's LDS sums; CARRY: every active
+// lane's line is finite, so each increment's start point is the previous one's
+// end point (see dda_increment_carried).
+template <bool EMIT, bool CARRY>
+__device__ inline void rt_replay_steps(RowLds<EMIT>& s, Dda& d, V2 p0, bool finite, uint32_t x, bool fx,
+                                       Counters* cnt) {
+    const int16_t rtx = div8(d.x), rty = div8(d.y);
+    bool ended = false;
+    for (int k = 0; k < 16; k++) {
+        float area, height;
+        dda_increment_carried(d, p0, CARRY || finite, area, height);
+        const uint32_t px = x * (uint32_t)UWS + (uint32_t)(((d.y & 7) << 3) | (d.x & 7));
+        if (fx) {
+            atomicAdd(reinterpret_cast<unsigned long long*>(&s.acc_a[px]), (unsigned long long)fx_of(area));
+            atomicAdd(reinterpret_cast<unsigned long long*>(&s.acc_h[px]), (unsigned long long)fx_of(height));
+        } else {
+            atomicAdd(reinterpret_cast<float*>(&s.acc_a[px]), area);
+            atomicAdd(reinterpret_cast<float*>(&s.acc_h[px]), height);
+        }
+        if (dda_step(d) || div8(d.x) != rtx || div8(d.y) != rty) {
+            ended = true;
+            break;
+        }
+    }
+    if (!ended) atomicOr(&cnt->error, (uint32_t)ERR_REPLAY);
+}
+
 // Replays a run (x: its tile within the window) into the window's LDS sums
-// with float atomics (arrival order).
+// (fixed-point or float atomics, arrival order).
 template <bool EMIT>
 __device__ inline void rt_replay_run(RowLds<EMIT>& s, uint32_t x, const RunRec& rr, const float4& L, bool fx,
                                      Counters* cnt) {
@@ -1304,27 +1332,11 @@ __device__ inline void rt_replay_run(RowLds<EMIT>& s, uint32_t x, const RunRec& 
         d.t0 = 0.0f;
         dda_step(d);
     }
-    const int16_t rtx = div8(d.x), rty = div8(d.y);
     const bool finite = line_finite(last, point);
-    V2 p0 = dda_point(d, d.t0);
-    bool ended = false;
-    for (int k = 0; k < 16; k++) {
-        float area, height;
-        dda_increment_carried(d, p0, finite, area, height);
-        const uint32_t px = x * (uint32_t)UWS + (uint32_t)(((d.y & 7) << 3) | (d.x & 7));
-        if (fx) {
-            atomicAdd(reinterpret_cast<unsigned long long*>(&s.acc_a[px]), (unsigned long long)fx_of(area));
-            atomicAdd(reinterpret_cast<unsigned long long*>(&s.acc_h[px]), (unsigned long long)fx_of(height));
-        } else {
-            atomicAdd(reinterpret_cast<float*>(&s.acc_a[px]), area);
-            atomicAdd(reinterpret_cast<float*>(&s.acc_h[px]), height);
-        }
-        if (dda_step(d) || div8(d.x) != rtx || div8(d.y) != rty) {
-            ended = true;
-            break;
-        }
-    }
-    if (!ended) atomicOr(&cnt->error, (uint32_t)ERR_REPLAY);
+    const V2 p0 = dda_point(d, d.t0);
+    // a wave-uniform choice: the start-point recomputation only where a line needs it
+    if (__ballot(!finite) == 0) rt_replay_steps<EMIT, true>(s, d, p0, finite, x, fx, cnt);
+    else rt_replay_steps<EMIT, false>(s, d, p0, finite, x, fx, cnt);
 }
 
 // RT_U entries per thread of a window (entries e0 + u * RT_NT), with their
