@@ -1266,14 +1266,14 @@ constexpr int RT_ROWS = TILE / RT_NW;  // pixel rows whose carry each wave takes
 static_assert(RT_ROWS * RT_NW == TILE && RT_ROWS <= 2, "the carry takes one or two pixel rows per wave");
 constexpr int RT_U = 2;  // entries per thread whose loads are issued together
 
-// The window's per-pixel sums are 64-bit fixed point (units of 2^-39): LDS
-// integer atomics cost a fraction of float ones, and the sums of the
-// increments (|area|, |height| <= 1 + a few ulps, each truncated at 2^-39) are
-// then exact, so that the decoded f32 sum is the correctly rounded sum of the
-// increments, independent of arrival order.  A window of a full-range path
-// (coordinates that may be huge or non-finite) or with 2^22 entries or more
-// keeps f32 atomics in the slots' low words instead.
-constexpr uint32_t RT_FX_MAX_ENTRIES = 1u << 22;  // 2^22 increments of |v| < 2 per pixel stay below 2^63
+// The window's per-pixel sums are fixed point in f64 slots (fx_of: units of
+// 2^-30, integer-valued): the sums of the increments (|area|, |height| <= 1 +
+// a few ulps, each truncated at 2^-30) are then exact, so that the decoded f32
+// sum is the correctly rounded sum of the increments, independent of arrival
+// order.  A window of a full-range path (coordinates that may be huge or
+// non-finite) or with 2^22 entries or more keeps f32 atomics in the slots' low
+// words instead.
+constexpr uint32_t RT_FX_MAX_ENTRIES = 1u << 22;  // a pixel takes at most one increment per run: < 2^22 x (1 + ulps) < 2^23
 
 template <bool EMIT>
 struct RowLds {
@@ -1300,8 +1300,8 @@ __device__ inline void rt_replay_steps(RowLds<EMIT>& s, Dda& d, V2 p0, bool fini
         dda_increment_carried(d, p0, CARRY || finite, area, height);
         const uint32_t px = x * (uint32_t)UWS + (uint32_t)(((d.y & 7) << 3) | (d.x & 7));
         if (fx) {
-            atomicAdd(reinterpret_cast<unsigned long long*>(&s.acc_a[px]), (unsigned long long)fx_of(area));
-            atomicAdd(reinterpret_cast<unsigned long long*>(&s.acc_h[px]), (unsigned long long)fx_of(height));
+            atomicAdd(reinterpret_cast<double*>(&s.acc_a[px]), fx_of(area));
+            atomicAdd(reinterpret_cast<double*>(&s.acc_h[px]), fx_of(height));
         } else {
             atomicAdd(reinterpret_cast<float*>(&s.acc_a[px]), area);
             atomicAdd(reinterpret_cast<float*>(&s.acc_h[px]), height);
@@ -1537,7 +1537,8 @@ __global__ __launch_bounds__(RT_NT) void k_row_tiles(RowTileArgs a) {
                         float* fh = reinterpret_cast<float*>(s.acc_h);  // the slots' low words
 #pragma unroll
                         for (int k = 0; k < 8; k++) {
-                            const float h = fx ? fx_to_f(s.acc_h[base + k]) : fh[2 * (base + k)];
+                            const float h = fx ? fx_to_f(reinterpret_cast<const double*>(s.acc_h)[base + k])
+                                               : fh[2 * (base + k)];
                             fh[2 * (base + k)] = tot;
                             tot += h;
                         }
@@ -1563,7 +1564,8 @@ __global__ __launch_bounds__(RT_NT) void k_row_tiles(RowTileArgs a) {
                     const float* fa = reinterpret_cast<const float*>(s.acc_a);
 #pragma unroll
                     for (int k = 0; k < 4; k++) {
-                        const float av = fx ? fx_to_f(s.acc_a[base + k]) : fa[2 * (base + k)];
+                        const float av = fx ? fx_to_f(reinterpret_cast<const double*>(s.acc_a)[base + k])
+                                            : fa[2 * (base + k)];
                         const float v = (cin + fh[2 * (base + k)]) + av;
                         word |= (uint32_t)fminf(fabsf(v) * 256.0f, 255.0f) << (8 * k);
                         if (a.tile_cov) a.tile_cov[ti * 64 + quad * 4 + (uint32_t)k] = fminf(fabsf(v), 1.0f);

@@ -168,23 +168,19 @@ struct BoxAcc {
     }
 };
 
-// Per-pixel sums in 64-bit fixed point, units of 2^-39 (the unordered tile
-// stage and the dense kernels): an increment's |area|, |height| <= 1 plus a
-// few ulps for lines with finite coordinates within +-30000, so 2^22 of them
-// sum below 2^63; LDS integer atomics cost a fraction of float ones, and the
-// integer sum is exact, so the decoded f32 value is the correctly rounded sum
-// of the increments (each truncated at 2^-39) in any arrival order.
-constexpr double FX_UNIT = 1.8189894035458565e-12;  // 2^-39
+// Per-pixel sums of the unordered tile stage in fixed point, units of 2^-30,
+// held as integer-valued f64: an increment's |area|, |height| <= 1 plus a few
+// ulps for lines with finite coordinates within +-30000, so the sums of 2^22 of
+// them stay below 2^53 units, where every f64 sum of integers is exact; the
+// sum is then independent of arrival order, and the decoded f32 value is the
+// correctly rounded sum of the increments (each truncated at 2^-30).  The
+// conversion is three f64 operations (f32 to f64, scale, truncate), where a
+// 64-bit integer one needs ten.
+constexpr double FX_SCALE = 1073741824.0;       // 2^30
+constexpr double FX_UNIT = 9.313225746154785e-10;  // 2^-30
 
-__device__ inline long long fx_of(float v) {
-    // v * 2^39 truncated toward zero, in two exact f32 steps: the integer
-    // part of v * 2^8, and the fraction scaled by 2^31
-    const float sv = v * 256.0f;
-    const float hi = truncf(sv);
-    const float lo = (sv - hi) * 2147483648.0f;
-    return (long long)(int)hi * 2147483648ll + (long long)(int)lo;
-}
-__device__ inline float fx_to_f(long long q) { return (float)((double)q * FX_UNIT); }
+__device__ inline double fx_of(float v) { return trunc((double)v * FX_SCALE); }
+__device__ inline float fx_to_f(double q) { return (float)(q * FX_UNIT); }
 
 // ------------------------------------------------------ (b) DDA binning
 //
