@@ -1293,12 +1293,16 @@ struct RowLds {
 template <bool EMIT, bool CARRY>
 __device__ inline void rt_replay_steps(RowLds<EMIT>& s, Dda& d, V2 p0, bool finite, uint32_t x, bool fx,
                                        Counters* cnt) {
-    const int16_t rtx = div8(d.x), rty = div8(d.y);
+    // the pixel within the run's tile: the run ends when it leaves 0..7 on
+    // either axis (or the line ends)
+    uint32_t lx = (uint32_t)(d.x & 7), ly = (uint32_t)(d.y & 7);
+    const uint32_t xb = x * (uint32_t)UWS;
+    const uint32_t lxd = (uint32_t)(int32_t)d.x_dir, lyd = (uint32_t)(int32_t)d.y_dir;
     bool ended = false;
     for (int k = 0; k < 16; k++) {
         float area, height;
         dda_increment_carried(d, p0, CARRY || finite, area, height);
-        const uint32_t px = x * (uint32_t)UWS + (uint32_t)(((d.y & 7) << 3) | (d.x & 7));
+        const uint32_t px = xb + ((ly << 3) | lx);
         if (fx) {
             atomicAdd(reinterpret_cast<double*>(&s.acc_a[px]), fx_of(area));
             atomicAdd(reinterpret_cast<double*>(&s.acc_h[px]), fx_of(height));
@@ -1306,7 +1310,21 @@ __device__ inline void rt_replay_steps(RowLds<EMIT>& s, Dda& d, V2 p0, bool fini
             atomicAdd(reinterpret_cast<float*>(&s.acc_a[px]), area);
             atomicAdd(reinterpret_cast<float*>(&s.acc_h[px]), height);
         }
-        if (dda_step(d) || div8(d.x) != rtx || div8(d.y) != rty) {
+        // dda_step, with the in-tile pixel moved alongside
+        float c;
+        if (d.row_t1 < d.col_t1) {
+            c = d.row_t1;
+            d.row_t1 = fminf(d.row_t1 + d.y_step, 1.0f);
+            d.y = wrap16(d.y + d.y_dir);
+            ly += lyd;
+        } else {
+            c = d.col_t1;
+            d.col_t1 = fminf(d.col_t1 + d.x_step, 1.0f);
+            d.x = wrap16(d.x + d.x_dir);
+            lx += lxd;
+        }
+        d.t0 = c;
+        if (c == 1.0f || (lx | ly) > 7u) {
             ended = true;
             break;
         }
