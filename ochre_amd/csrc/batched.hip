@@ -914,7 +914,7 @@ int run_batched(Context* ctx, const SegInput& in, OutputDesc& out, hipStream_t s
                 hipLaunchKernelGGL((k_dense<2, 32>), dim3(n_small), dim3(2 * WAVE), 0, st, f, fo, ctl, queues[i],
                                    lists[i], counts[i]);
             else if (stages[i] == D_LARGE)
-                hipLaunchKernelGGL((k_dense<16, 256>), dim3(n_large), dim3(16 * WAVE), 0, st, f, fo, ctl, queues[i],
+                hipLaunchKernelGGL((k_dense<8, 256>), dim3(n_large), dim3(8 * WAVE), 0, st, f, fo, ctl, queues[i],
                                    lists[i], counts[i]);
             else if (dnw == 2)
                 hipLaunchKernelGGL((k_dense<2, 64>), dim3(n_groups), dim3(2 * WAVE), 0, st, f, fo, ctl, queues[i],
